@@ -1,0 +1,635 @@
+// kr_api.hip -- host side of the C-ABI (include/kratos_amd.h): context,
+// block registry, device pools, AoS <-> SoA packing, launch sequencing and
+// the hipGraph replay of whole cycles.
+#include "../../include/kratos_amd.h"
+#include "kr_staged.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace kr {
+
+__global__ void k_cycle_begin(TimeState *ts) {
+    // cycle::step stop rule (mesh/cycle/cycle.cpp:143-144)
+    ts->skip = (ts->t >= ts->t_lim) ? 1 : 0;
+    if (!ts->skip) ts->cycles += 1;
+}
+
+// Global CFL reduction and the dt / t bookkeeping of redc_dt_start/finish
+// (mesh/cycle/cycle.cpp:77-115): p_dt = min(p_dt, CFL), dt_use clamped to
+// t_lim when topping up, t advanced, next seed = dt_use * dt_expand.
+__global__ void __launch_bounds__(1024) k_dt_finalize(TimeState *ts, const double *part, int npart) {
+    if (ts->skip) return;
+    __shared__ double red[16];
+    double v = (double)FLT_MAX;
+    for (int i = threadIdx.x; i < npart; i += blockDim.x) v = pmin(v, part[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = pmin(v, red[i]);
+        const double p = __builtin_fmin(v, ts->p_dt);
+        const double dt_use = ts->top_up ? (p < ts->t_lim - ts->t ? p : ts->t_lim - ts->t) : p;
+        if (ts->top_up) {
+            const double tn = dt_use + ts->t;
+            ts->t = tn < ts->t_lim ? tn : ts->t_lim;
+        } else {
+            ts->t = dt_use + ts->t;
+        }
+        ts->dt_use = dt_use;
+        ts->p_dt = dt_use * ts->dt_expand;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_max_reduce(const double *part, int npart, double *out) {
+    __shared__ double red[16];
+    double v = 0;
+    for (int i = threadIdx.x; i < npart; i += blockDim.x) v = pmax(v, part[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = pmax(v, red[i]);
+        *out = v;
+    }
+}
+
+}  // namespace kr
+
+using namespace kr;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define KR_HIP(call)                                                               \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            throw HipError(std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct kr_ctx {
+    kr_config cfg{};
+    std::string err;
+    std::vector<kr_block_desc> descs;
+    std::unordered_map<int, int> id2idx;
+    bool committed = false;
+    bool mhd = false, mixed = false;
+    int n[3] = {0, 0, 0};
+    int ndim = 0, nv = 5;
+    size_t ncell = 0, nface = 0, nfa[3] = {0, 0, 0};
+    // device buffers
+    double *U = nullptr, *BF = nullptr;
+    void *W = nullptr, *F[3] = {nullptr, nullptr, nullptr}, *EE = nullptr;
+    BlockGeo *geo = nullptr;
+    TimeState *ts = nullptr;
+    double *dt_part = nullptr, *red_part = nullptr, *red_out = nullptr;
+    size_t n_dt_part = 0, n_red_part = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    kr_stats stats{};
+    std::vector<double> stage;   // host staging for packing
+
+    template <class FT> MeshView<FT> view() const {
+        MeshView<FT> m{};
+        for (int a = 0; a < 3; ++a) m.n[a] = n[a];
+        m.ndim = ndim;
+        m.nblocks = (int)descs.size();
+        m.nv = nv;
+        for (int s = 0; s < 6; ++s) m.bnd[s] = cfg.bnd[s];
+        m.gamma = cfg.gamma;
+        m.cfl = cfg.cfl;
+        m.d_floor = cfg.rho_floor;
+        m.t_floor = cfg.tmp_floor;
+        m.ncell = ncell;
+        m.nface = nface;
+        for (int a = 0; a < 3; ++a) m.nfa[a] = nfa[a];
+        m.U = U;
+        m.BF = BF;
+        m.W = (FT *)W;
+        for (int a = 0; a < 3; ++a) m.F[a] = (FT *)F[a];
+        m.EE = (FT *)EE;
+        m.geo = geo;
+        m.ts = ts;
+        m.dt_part = dt_part;
+        return m;
+    }
+};
+
+namespace {
+
+void release(kr_ctx *c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    void *bufs[] = {c->U, c->BF, c->W, c->F[0], c->F[1], c->F[2], c->EE,
+                    c->geo, c->ts, c->dt_part, c->red_part, c->red_out};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    c->U = c->BF = nullptr;
+    c->W = c->EE = nullptr;
+    c->F[0] = c->F[1] = c->F[2] = nullptr;
+    c->geo = nullptr;
+    c->ts = nullptr;
+    c->dt_part = c->red_part = c->red_out = nullptr;
+}
+
+template <class T> T *dalloc(size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    KR_HIP(hipMalloc(&p, count * sizeof(T)));
+    return (T *)p;
+}
+
+// Lay out the pools once every block is known (the reference allocates per
+// block in dual_t::setup; here each field is one pool, block-major).
+void commit(kr_ctx *c) {
+    if (c->committed) return;
+    if (c->descs.empty()) throw std::invalid_argument("no blocks registered");
+    const size_t nb = c->descs.size();
+    for (int a = 0; a < 3; ++a) c->n[a] = c->cfg.n_cell_block[a];
+    c->ndim = 0;
+    for (int a = 0; a < 3; ++a) c->ndim += c->n[a] > 1 ? 1 : 0;
+    c->ncell = (size_t)c->n[0] * c->n[1] * c->n[2];
+    c->nface = (size_t)(c->n[0] + 1) * (c->n[1] + 1) * (c->n[2] + 1);
+    for (int ax = 0; ax < 3; ++ax) {
+        size_t s = 1;
+        for (int a = 0; a < 3; ++a) s *= (size_t)(c->n[a] + (a == ax ? 1 : 2));
+        c->nfa[ax] = s;
+    }
+    const size_t fs = c->mixed ? sizeof(float) : sizeof(double);
+    c->U = dalloc<double>(nb * 2 * 5 * c->ncell);
+    if (c->mhd) c->BF = dalloc<double>(nb * 2 * 3 * c->nface);
+    KR_HIP(hipMalloc(&c->W, nb * c->nv * c->ncell * fs));
+    for (int a = 0; a < c->ndim; ++a) KR_HIP(hipMalloc(&c->F[a], nb * c->nv * c->nfa[a] * fs));
+    if (c->mhd) KR_HIP(hipMalloc(&c->EE, nb * 3 * c->nface * fs));
+    KR_HIP(hipMemset(c->U, 0, nb * 2 * 5 * c->ncell * sizeof(double)));
+    if (c->mhd) KR_HIP(hipMemset(c->BF, 0, nb * 2 * 3 * c->nface * sizeof(double)));
+    KR_HIP(hipMemset(c->W, 0, nb * c->nv * c->ncell * fs));
+
+    std::vector<BlockGeo> g(nb);
+    for (size_t b = 0; b < nb; ++b) {
+        const kr_block_desc &d = c->descs[b];
+        BlockGeo &G = g[b];
+        std::memset(&G, 0, sizeof(G));
+        for (int a = 0; a < 3; ++a) G.dx[a] = d.dx0[a];
+        // meshgen.cpp:263-283: products in the reference's order
+        for (int a = 0; a < 3; ++a) {
+            double s = 1.;
+            for (int m = 1; m < 3; ++m) s *= G.dx[(a + m) % 3];
+            G.sfc[a] = s;
+        }
+        double v = 1.;
+        for (int a = 0; a < 3; ++a) v *= G.dx[a];
+        G.vol = v;
+        G.phy = 0;
+        for (int k = 0; k < 27; ++k) {
+            const int id = d.nbr[k];
+            if (id < 0) {
+                G.nbr[k] = -1;
+                continue;
+            }
+            auto it = c->id2idx.find(id);
+            if (it == c->id2idx.end())
+                throw std::invalid_argument("neighbour id " + std::to_string(id) + " not registered");
+            G.nbr[k] = it->second;
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int s = 0; s < 2; ++s) {
+                if (c->n[a] < 2) continue;
+                int dir[3] = {0, 0, 0};
+                dir[a] = s ? 1 : -1;
+                const int k = ((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1);
+                if (G.nbr[k] < 0) G.phy |= 1 << (2 * a + s);
+            }
+        if (c->mhd && G.phy)
+            throw std::invalid_argument("MHD supports periodic / internal sides only");
+    }
+    c->geo = dalloc<BlockGeo>(nb);
+    KR_HIP(hipMemcpy(c->geo, g.data(), nb * sizeof(BlockGeo), hipMemcpyHostToDevice));
+
+    TimeState ts{};
+    ts.t = c->cfg.t_0;
+    ts.t_lim = c->cfg.t_lim;
+    ts.p_dt = c->cfg.dt_init < FLT_MIN ? (double)(FLT_MAX / 10) : c->cfg.dt_init;
+    ts.dt_use = c->cfg.dt_init;
+    ts.dt_expand = c->cfg.dt_expand;
+    ts.top_up = c->cfg.top_up;
+    c->ts = dalloc<TimeState>(1);
+    KR_HIP(hipMemcpy(c->ts, &ts, sizeof(ts), hipMemcpyHostToDevice));
+
+    c->n_dt_part = nb * ceil_div(c->ncell, kThreads);
+    c->dt_part = dalloc<double>(c->n_dt_part);
+    c->n_red_part = c->n_dt_part;
+    c->red_part = dalloc<double>(c->n_red_part);
+    c->red_out = dalloc<double>(1);
+    c->stats.n_blocks = (int)nb;
+    c->stats.n_cells = (long long)(nb * c->ncell);
+    // 5 state passes per Heun cycle of (5 conserved + 3 face-B) FP64 values
+    c->stats.alg_bytes_per_cell_update = 5.0 * (5 + (c->mhd ? 3 : 0)) * 8.0;
+    c->committed = true;
+}
+
+// -------------------------------------------------------------- launches
+struct Launcher {
+    kr_ctx *c;
+    bool prof = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+
+    template <class F> void run(int kind, F &&f) {
+        if (!prof) {
+            f();
+            return;
+        }
+        hipEvent_t a, b;
+        KR_HIP(hipEventCreate(&a));
+        KR_HIP(hipEventCreate(&b));
+        KR_HIP(hipEventRecord(a, c->stream));
+        f();
+        KR_HIP(hipEventRecord(b, c->stream));
+        evs.push_back({kind, {a, b}});
+    }
+
+    template <class FT, bool MHD> void stage(int step) {
+        const MeshView<FT> m = c->view<FT>();
+        const unsigned nb = (unsigned)c->descs.size();
+        const dim3 blk(kThreads);
+        run(KR_K_C2P, [&] {
+            hipLaunchKernelGGL((k_c2p<FT, MHD>), dim3(ceil_div(c->ncell, kThreads), nb), blk, 0,
+                               c->stream, m, step);
+        });
+        if (step == 0)
+            run(KR_K_DTFIN, [&] {
+                hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1024), 0, c->stream, c->ts,
+                                   (const double *)c->dt_part, (int)c->n_dt_part);
+            });
+        for (int ax = c->ndim - 1; ax >= 0; --ax)
+            run(KR_K_FLUX_X + ax, [&] {
+                hipLaunchKernelGGL((k_flux<FT, MHD>), dim3(ceil_div(c->nfa[ax], kThreads), nb), blk,
+                                   0, c->stream, m, ax, step);
+            });
+        if constexpr (MHD)
+            run(KR_K_EMF, [&] {
+                hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), nb), blk, 0,
+                                   c->stream, m);
+            });
+        run(KR_K_UPDATE, [&] {
+            hipLaunchKernelGGL((k_update<FT, MHD>),
+                               dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), nb), blk, 0,
+                               c->stream, m, step);
+        });
+    }
+
+    template <class FT, bool MHD> void cycle_t() {
+        run(KR_K_BEGIN, [&] {
+            hipLaunchKernelGGL(k_cycle_begin, dim3(1), dim3(1), 0, c->stream, c->ts);
+        });
+        stage<FT, MHD>(0);
+        stage<FT, MHD>(1);
+    }
+
+    void cycle() {
+        if (c->mixed) {
+            if (c->mhd) cycle_t<float, true>();
+            else cycle_t<float, false>();
+        } else {
+            if (c->mhd) cycle_t<double, true>();
+            else cycle_t<double, false>();
+        }
+        KR_HIP(hipGetLastError());
+    }
+};
+
+int fail(kr_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F> int guarded(kr_ctx *c, F &&f) {
+    if (!c) return KR_EINVAL;
+    try {
+        return f();
+    } catch (const HipError &e) {
+        return fail(c, KR_EHIP, e.what());
+    } catch (const std::invalid_argument &e) {
+        return fail(c, KR_EINVAL, e.what());
+    } catch (const std::bad_alloc &e) {
+        return fail(c, KR_ENOMEM, e.what());
+    } catch (const std::exception &e) {
+        return fail(c, KR_ESTATE, e.what());
+    }
+}
+
+TimeState read_ts(kr_ctx *c) {
+    TimeState ts{};
+    KR_HIP(hipMemcpy(&ts, c->ts, sizeof(ts), hipMemcpyDeviceToHost));
+    return ts;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kr_create(const kr_config *cfg, kr_ctx **out) {
+    if (!cfg || !out) return KR_EINVAL;
+    *out = nullptr;
+    auto *c = new (std::nothrow) kr_ctx();
+    if (!c) return KR_ENOMEM;
+    c->cfg = *cfg;
+    c->mhd = cfg->model == KR_MHD;
+    c->mixed = cfg->precision == KR_MIXED;
+    c->nv = c->mhd ? 8 : 5;
+    std::string why;
+    if (cfg->model != KR_HYDRO && cfg->model != KR_MHD) why = "unknown model";
+    else if (cfg->precision != KR_FP64 && cfg->precision != KR_MIXED) why = "unknown precision";
+    else if (!(cfg->gamma > 1)) why = "gamma must exceed 1";
+    else if (!(cfg->cfl > 0)) why = "cfl must be positive";
+    for (int a = 0; a < 3 && why.empty(); ++a) {
+        if (cfg->n_cell_block[a] < 1) why = "n_cell_block must be >= 1";
+        if (c->mhd && cfg->n_cell_block[a] < 2) why = "MHD needs 3-D blocks";
+        if ((cfg->bnd[2 * a] == KR_PER) != (cfg->bnd[2 * a + 1] == KR_PER))
+            why = "periodic must be set on both sides of an axis";
+    }
+    for (int s = 0; s < 6 && why.empty(); ++s)
+        if (cfg->bnd[s] < KR_PER || cfg->bnd[s] > KR_REF) why = "unknown boundary kind";
+    if (!why.empty()) {
+        delete c;
+        return KR_EINVAL;
+    }
+    int rc = guarded(c, [&] {
+        int ndev = 0;
+        KR_HIP(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw HipError("no HIP device");
+        KR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        KR_HIP(hipEventCreate(&c->ev0));
+        KR_HIP(hipEventCreate(&c->ev1));
+        return KR_OK;
+    });
+    if (rc != KR_OK) {
+        fprintf(stderr, "kr_create: %s\n", c->err.c_str());
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return KR_OK;
+}
+
+int kr_add_blocks(kr_ctx *c, int n, const kr_block_desc *blocks) {
+    return guarded(c, [&] {
+        if (c->committed) throw std::runtime_error("blocks must be added before the first upload/step");
+        if (n < 0 || (n > 0 && !blocks)) throw std::invalid_argument("bad block list");
+        for (int i = 0; i < n; ++i) {
+            const kr_block_desc &d = blocks[i];
+            for (int a = 0; a < 3; ++a)
+                if (d.n_ceff[a] != c->cfg.n_cell_block[a])
+                    throw std::invalid_argument("block size differs from n_cell_block");
+            if (d.level != 0 && d.level != blocks[0].level)
+                throw std::invalid_argument("only same-level meshes are supported");
+            if (c->id2idx.count(d.id)) throw std::invalid_argument("duplicate block id");
+            c->id2idx[d.id] = (int)c->descs.size();
+            c->descs.push_back(d);
+        }
+        return KR_OK;
+    });
+}
+
+int kr_upload(kr_ctx *c, int id, const double *u_aos, const double *bf_aos) {
+    return guarded(c, [&] {
+        commit(c);
+        auto it = c->id2idx.find(id);
+        if (it == c->id2idx.end()) throw std::invalid_argument("unknown block id");
+        if (!u_aos || (c->mhd && !bf_aos)) throw std::invalid_argument("null state pointer");
+        const size_t b = it->second;
+        const int nvar = c->mhd ? 8 : 5;
+        std::vector<double> &s = c->stage;
+        s.assign(5 * c->ncell, 0.0);
+        for (size_t cc = 0; cc < c->ncell; ++cc)
+            for (int v = 0; v < 5; ++v) s[v * c->ncell + cc] = u_aos[cc * nvar + v];
+        KR_HIP(hipMemcpy(c->U + b * 2 * 5 * c->ncell, s.data(), 5 * c->ncell * sizeof(double),
+                         hipMemcpyHostToDevice));
+        if (c->mhd) {
+            const int N0 = c->n[0] + 5, N1 = c->n[1] + 5;
+            s.assign(3 * c->nface, 0.0);
+            for (int k = 0; k <= c->n[2]; ++k)
+                for (int j = 0; j <= c->n[1]; ++j)
+                    for (int i = 0; i <= c->n[0]; ++i) {
+                        const size_t src = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
+                        const size_t f = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                        for (int a = 0; a < 3; ++a) s[a * c->nface + f] = bf_aos[src + a];
+                    }
+            KR_HIP(hipMemcpy(c->BF + b * 2 * 3 * c->nface, s.data(), 3 * c->nface * sizeof(double),
+                             hipMemcpyHostToDevice));
+        }
+        return KR_OK;
+    });
+}
+
+int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
+    return guarded(c, [&] {
+        commit(c);
+        auto it = c->id2idx.find(id);
+        if (it == c->id2idx.end()) throw std::invalid_argument("unknown block id");
+        if (!u_aos) throw std::invalid_argument("null state pointer");
+        KR_HIP(hipStreamSynchronize(c->stream));
+        const size_t b = it->second;
+        const int nvar = c->mhd ? 8 : 5;
+        std::vector<double> s(5 * c->ncell);
+        KR_HIP(hipMemcpy(s.data(), c->U + b * 2 * 5 * c->ncell, 5 * c->ncell * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        for (size_t cc = 0; cc < c->ncell; ++cc)
+            for (int v = 0; v < 5; ++v) u_aos[cc * nvar + v] = s[v * c->ncell + cc];
+        if (c->mhd) {
+            std::vector<double> f(3 * c->nface);
+            KR_HIP(hipMemcpy(f.data(), c->BF + b * 2 * 3 * c->nface, 3 * c->nface * sizeof(double),
+                             hipMemcpyDeviceToHost));
+            const size_t sh[3] = {1, (size_t)(c->n[0] + 1), (size_t)(c->n[0] + 1) * (c->n[1] + 1)};
+            for (int k = 0; k < c->n[2]; ++k)
+                for (int j = 0; j < c->n[1]; ++j)
+                    for (int i = 0; i < c->n[0]; ++i) {
+                        const size_t cc = ((size_t)k * c->n[1] + j) * c->n[0] + i;
+                        const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                        for (int a = 0; a < 3; ++a)
+                            u_aos[cc * nvar + BCC + a] =
+                                (f[a * c->nface + f0] + f[a * c->nface + f0 + sh[a]]) / 2;
+                    }
+            if (bf_aos) {
+                const int N0 = c->n[0] + 5, N1 = c->n[1] + 5, N2 = c->n[2] + 5;
+                std::memset(bf_aos, 0, (size_t)N0 * N1 * N2 * 3 * sizeof(double));
+                for (int k = 0; k <= c->n[2]; ++k)
+                    for (int j = 0; j <= c->n[1]; ++j)
+                        for (int i = 0; i <= c->n[0]; ++i) {
+                            const size_t dst = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
+                            const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                            for (int a = 0; a < 3; ++a) bf_aos[dst + a] = f[a * c->nface + f0];
+                        }
+            }
+        }
+        return KR_OK;
+    });
+}
+
+int kr_step(kr_ctx *c, double *dt_taken) {
+    return guarded(c, [&] {
+        commit(c);
+        const TimeState before = read_ts(c);
+        KR_HIP(hipEventRecord(c->ev0, c->stream));
+        Launcher L{c};
+        L.cycle();
+        KR_HIP(hipEventRecord(c->ev1, c->stream));
+        KR_HIP(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const TimeState after = read_ts(c);
+        c->stats.gpu_ms = ms;
+        c->stats.cycles = after.cycles - before.cycles;
+        if (dt_taken) *dt_taken = c->stats.cycles ? after.dt_use : 0.0;
+        return KR_OK;
+    });
+}
+
+int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
+    return guarded(c, [&] {
+        commit(c);
+        if (n_cycles < 0) throw std::invalid_argument("n_cycles < 0");
+        if (!c->graph) {
+            hipGraph_t g;
+            KR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            Launcher L{c};
+            L.cycle();
+            KR_HIP(hipStreamEndCapture(c->stream, &g));
+            KR_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+            KR_HIP(hipGraphDestroy(g));
+        }
+        const TimeState before = read_ts(c);
+        KR_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int i = 0; i < n_cycles; ++i) KR_HIP(hipGraphLaunch(c->graph, c->stream));
+        KR_HIP(hipEventRecord(c->ev1, c->stream));
+        KR_HIP(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const TimeState after = read_ts(c);
+        c->stats.gpu_ms = ms;
+        c->stats.cycles = after.cycles - before.cycles;
+        if (n_done) *n_done = c->stats.cycles;
+        return KR_OK;
+    });
+}
+
+int kr_profile(kr_ctx *c, int n_cycles) {
+    return guarded(c, [&] {
+        commit(c);
+        Launcher L{c, true};
+        for (int i = 0; i < n_cycles; ++i) L.cycle();
+        KR_HIP(hipStreamSynchronize(c->stream));
+        double sum[8] = {0};
+        long long cnt[8] = {0};
+        for (auto &e : L.evs) {
+            float ms = 0;
+            KR_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+            sum[e.first] += ms;
+            cnt[e.first] += 1;
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        for (int k = 0; k < 8; ++k) {
+            c->stats.kernel_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+            c->stats.kernel_calls[k] = cnt[k];
+        }
+        return KR_OK;
+    });
+}
+
+int kr_get_time(kr_ctx *c, double *t, double *dt_last, double *dt_seed) {
+    return guarded(c, [&] {
+        commit(c);
+        KR_HIP(hipStreamSynchronize(c->stream));
+        const TimeState ts = read_ts(c);
+        if (t) *t = ts.t;
+        if (dt_last) *dt_last = ts.dt_use;
+        if (dt_seed) *dt_seed = ts.p_dt;
+        return KR_OK;
+    });
+}
+
+int kr_set_time(kr_ctx *c, double t, double dt_seed) {
+    return guarded(c, [&] {
+        commit(c);
+        KR_HIP(hipStreamSynchronize(c->stream));
+        TimeState ts = read_ts(c);
+        ts.t = t;
+        ts.p_dt = dt_seed < FLT_MIN ? (double)(FLT_MAX / 10) : dt_seed;
+        KR_HIP(hipMemcpy(c->ts, &ts, sizeof(ts), hipMemcpyHostToDevice));
+        return KR_OK;
+    });
+}
+
+int kr_divb_max(kr_ctx *c, double *divb) {
+    return guarded(c, [&] {
+        commit(c);
+        if (!divb) throw std::invalid_argument("null output");
+        if (!c->mhd) {
+            *divb = 0;
+            return KR_OK;
+        }
+        const unsigned nb = (unsigned)c->descs.size();
+        const unsigned gx = ceil_div(c->ncell, kThreads);
+        if (c->mixed)
+            hipLaunchKernelGGL(k_divb<float>, dim3(gx, nb), dim3(kThreads), 0, c->stream,
+                               c->view<float>(), c->red_part);
+        else
+            hipLaunchKernelGGL(k_divb<double>, dim3(gx, nb), dim3(kThreads), 0, c->stream,
+                               c->view<double>(), c->red_part);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, c->stream,
+                           (const double *)c->red_part, (int)(gx * nb), c->red_out);
+        KR_HIP(hipGetLastError());
+        KR_HIP(hipMemcpyAsync(divb, c->red_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        KR_HIP(hipStreamSynchronize(c->stream));
+        return KR_OK;
+    });
+}
+
+int kr_get_stats(kr_ctx *c, kr_stats *out) {
+    if (!c || !out) return KR_EINVAL;
+    *out = c->stats;
+    return KR_OK;
+}
+
+int kr_sync(kr_ctx *c) {
+    return guarded(c, [&] {
+        KR_HIP(hipStreamSynchronize(c->stream));
+        return KR_OK;
+    });
+}
+
+const char *kr_last_error(kr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void kr_destroy(kr_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

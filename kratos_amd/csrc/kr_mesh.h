@@ -1,0 +1,61 @@
+// kr_mesh.h -- device-side mesh description shared by the kernels and the
+// host library.  Plain structs only: they are copied by value into kernel
+// arguments.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace kr {
+
+// Per-block geometry and topology, resident in HBM (one entry per block).
+// Uniform Cartesian blocks: the reference's geo_t with dx0 / sfc0 / vol0
+// (mesh/geometry/geometry.h:236-245), plus the 27-entry same-level
+// neighbour table that replaces the reference's ghost-exchange queues.
+struct BlockGeo {
+    double dx[3];      // dx0
+    double sfc[3];     // sfc0 (face areas, meshgen.cpp:263-272 product order)
+    double vol;        // vol0
+    int nbr[27];       // block index at ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical side
+    int phy;           // bit (2*axis + side) set on a physical side
+    int pad;
+};
+
+// Time / dt bookkeeping kept on the device so that a whole cycle (and a
+// multi-cycle hipGraph) runs without host round trips.  Mirrors
+// mesh/cycle/cycle.cpp:77-115 and :124-150.
+struct TimeState {
+    double t;          // simulation time
+    double t_lim;
+    double p_dt;       // device dt seed: min(prev * dt_expand, CFL)
+    double dt_use;     // dt actually taken in the current cycle
+    double dt_expand;
+    int top_up;
+    int skip;          // 1 once t >= t_lim at the start of a cycle
+    int cycles;        // cycles actually performed
+    int dt_stale;      // 1 when the CFL partials were not produced yet
+};
+
+// Device view of one mesh (all blocks share one block size).
+template <class FT>
+struct MeshView {
+    int n[3];          // cells per block along each axis
+    int ndim;          // axes with more than one cell
+    int nblocks;
+    int nv;            // variables in W and F: 5 hydro, 8 MHD
+    int bnd[6];        // boundary kind per side (0 per, 1 out, 2 fre, 3 ref)
+    double gamma, cfl, d_floor, t_floor;
+    size_t ncell;      // n0*n1*n2
+    size_t nface;      // (n0+1)(n1+1)(n2+1): face / edge box
+    size_t nfa[3];     // flux-box size for each sweep axis
+    double *U;         // [block][slot 2][var 5][ncell]
+    double *BF;        // [block][slot 2][comp 3][nface]          (MHD)
+    FT *W;             // [block][var nv][ncell]
+    FT *F[3];          // [block][var nv][nfa[a]]
+    FT *EE;            // [block][comp 3][nface]                   (MHD)
+    const BlockGeo *geo;
+    TimeState *ts;
+    double *dt_part;   // one CFL partial per c2p workgroup
+};
+
+}  // namespace kr

@@ -1,0 +1,434 @@
+// kr_staged.h -- the staged Godunov kernels: one launch per stage step, each
+// launch covering every block of the mesh (grid.y = block), SoA state in
+// HBM, neighbour-table reads instead of ghost copies.
+//
+// Per RK substep: k_c2p -> (k_dt_finalize at substep 0) -> k_flux x ndim
+// -> k_emf (MHD) -> k_update.  This is the reference's integrate() pipeline
+// (dynamics/dynamics.h:60-100) without per-block launches, streams, host
+// syncs or halo-copy kernels.
+#pragma once
+
+#include "kr_mesh.h"
+#include "kr_physics.h"
+
+namespace kr {
+
+constexpr int kThreads = 256;
+
+// ------------------------------------------------------------------ index
+__device__ __forceinline__ size_t cidx(const int *n, int i, int j, int k) {
+    return ((size_t)k * n[1] + j) * n[0] + i;
+}
+__device__ __forceinline__ size_t fidx(const int *n, int i, int j, int k) {
+    return ((size_t)k * (n[1] + 1) + j) * (n[0] + 1) + i;
+}
+// flux box of sweep axis `ax`: n+1 faces along ax, n+2 (one ghost line on
+// each side) along the transverse axes.
+__device__ __forceinline__ size_t xidx(const int *n, int ax, const int *p) {
+    const int d0 = n[0] + (ax == 0 ? 1 : 2), d1 = n[1] + (ax == 1 ? 1 : 2);
+    const int o0 = ax == 0 ? 0 : 1, o1 = ax == 1 ? 0 : 1, o2 = ax == 2 ? 0 : 1;
+    return ((size_t)(p[2] + o2) * d1 + (p[1] + o1)) * d0 + (p[0] + o0);
+}
+
+template <class FT>
+__device__ __forceinline__ int nbr_dir(const MeshView<FT> &m, const int *d) {
+    return ((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1);
+}
+
+// Load the primitive vector of cell p of block b.  Cells outside the block
+// come from the same-level neighbour (the reference fills them by the
+// ghost exchange, hydro/boundary/comm_functions.h:304-327); on a physical
+// side the reference's boundary functor is applied on the fly
+// (hydro/boundary/phys_functors.h:10-89).
+template <class FT, int NV>
+__device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const int *pin, FT *w) {
+    int p[3] = {pin[0], pin[1], pin[2]};
+    int d[3];
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        d[a] = p[a] < 0 ? -1 : (p[a] >= m.n[a] ? 1 : 0);
+        out |= d[a] != 0;
+    }
+    int blk = b;
+    int flip = -1;
+    if (out) {
+        const int nb = m.geo[b].nbr[nbr_dir(m, d)];
+        if (nb >= 0) {
+            blk = nb;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
+        } else {
+            // physical side: one axis out of range (hydro face ghosts)
+            int ax = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (d[a] != 0) ax = a;
+            const int side = d[ax] > 0;
+            const int kind = m.bnd[2 * ax + side];
+            const int g = side ? p[ax] - m.n[ax] : -1 - p[ax];
+            if (kind == 3) {                       // reflect
+                p[ax] = side ? m.n[ax] - 1 - g : g;
+                flip = ax;
+            } else {                               // outflow / free
+                p[ax] = side ? m.n[ax] - 1 : 0;
+                if (kind == 1) flip = 3 + 2 * ax + side;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] = p[a] < 0 ? 0 : (p[a] >= m.n[a] ? m.n[a] - 1 : p[a]);
+        }
+    }
+    const FT *src = m.W + (size_t)blk * NV * m.ncell + cidx(m.n, p[0], p[1], p[2]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) w[v] = src[(size_t)v * m.ncell];
+    if (flip >= 0 && flip < 3) {
+        w[MOM + flip] = w[MOM + flip] * -1;
+    } else if (flip >= 3) {                        // outflow inflow clamp
+        const int ax = (flip - 3) >> 1, side = (flip - 3) & 1;
+        if (w[MOM + ax] * (side ? -1 : 1) > 0) w[MOM + ax] = 0;
+    }
+}
+
+// Normal face field of block b at face p (component c); transverse indices
+// may sit one row outside the block (MHD transverse ghost lines), which the
+// reference fills by the tangential face-field exchange
+// (mhd/boundary/comm_functions.h:433-465).
+template <class FT>
+__device__ __forceinline__ double load_bf(const MeshView<FT> &m, int b, int slot, int c,
+                                          const int *pin) {
+    int p[3] = {pin[0], pin[1], pin[2]};
+    int d[3] = {0, 0, 0};
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a == c) continue;
+        d[a] = p[a] < 0 ? -1 : (p[a] >= m.n[a] ? 1 : 0);
+        out |= d[a] != 0;
+    }
+    int blk = b;
+    if (out) {
+        blk = m.geo[b].nbr[nbr_dir(m, d)];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
+    }
+    return m.BF[(((size_t)blk * 2 + slot) * 3 + c) * m.nface + fidx(m.n, p[0], p[1], p[2])];
+}
+
+// ------------------------------------------------------------- time state
+__global__ void k_cycle_begin(TimeState *ts);
+__global__ void k_dt_finalize(TimeState *ts, const double *part, int npart);
+
+// Wave64 + workgroup min reduction of a double; lane 0 of the workgroup
+// returns the result.
+__device__ __forceinline__ double block_min(double v) {
+    __shared__ double red[kThreads / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = pmin(v, red[i]);
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------- c2p
+// hydro/eos/adiabatic.h:211-259 and mhd/eos/adiabatic.h:127-235: primitives
+// with floors written back to the conserved slot, cell-centred B from the
+// face average, and at substep 0 the per-cell CFL dt reduced per workgroup.
+template <class FT, bool MHD>
+__global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
+    if (m.ts->skip) return;
+    constexpr int NV = MHD ? 8 : 5;
+    const int b = blockIdx.y;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dtc = (double)FLT_MAX;
+    if (c < m.ncell) {
+        const int i = (int)(c % m.n[0]);
+        const int j = (int)((c / m.n[0]) % m.n[1]);
+        const int k = (int)(c / ((size_t)m.n[0] * m.n[1]));
+        double *U = m.U + ((size_t)b * 2 + step) * 5 * m.ncell + c;
+        double cons[5], prim[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) cons[v] = U[(size_t)v * m.ncell];
+        const EosPar e{m.gamma, m.cfl, m.d_floor, m.t_floor};
+        const BlockGeo &g = m.geo[b];
+        FT *W = m.W + (size_t)b * NV * m.ncell + c;
+        bool hr, he;
+        if constexpr (MHD) {
+            const double *BF = m.BF + ((size_t)b * 2 + step) * 3 * m.nface;
+            double bc[3], bc_2[3], b_max[3];
+            double e_m(0);
+            const size_t f0 = fidx(m.n, i, j, k);
+            const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double b_l = BF[(size_t)a * m.nface + f0];
+                const double b_r = BF[(size_t)a * m.nface + f0 + sh[a]];
+                const double bc_m = (b_l + b_r) / 2;
+                bc[a] = bc_m;
+                const double bcs2 = bc_m * bc_m;
+                e_m += bcs2;
+                bc_2[a] = bcs2;
+                b_max[a] = pmax(kfabs(b_l), kfabs(b_r));
+            }
+            e_m /= 2;
+            cons[ENE] -= e_m;
+            c2p(cons, prim, e, hr, he);
+            cons[ENE] += e_m;
+            U[(size_t)ENE * m.ncell] = cons[ENE];
+            if (hr) U[0] = cons[RHO];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) W[(size_t)(BCC + a) * m.ncell] = (FT)bc[a];
+            if (step == 0) dtc = dt_mhd(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl);
+        } else {
+            const double cs2 = c2p(cons, prim, e, hr, he);
+            if (hr) U[0] = cons[RHO];
+            if (he) U[(size_t)ENE * m.ncell] = cons[ENE];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
+            if (step == 0) dtc = dt_hydro(prim, cs2, g.dx, m.ndim, m.cfl);
+        }
+    }
+    if (step == 0) {
+        const double r = block_min(dtc);
+        if (threadIdx.x == 0) m.dt_part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+    }
+}
+
+// ------------------------------------------------------------------ flux
+// One thread per face of the sweep along `axis` (PLM + HLLC / HLLD), the
+// flux_kernel of prototypes/integrator.h:14-32 with hydro/integrator/data.h
+// :334-352 (hydro) or mhd/integrator/data.h:53-141 (MHD lines, including the
+// transverse ghost lines) for the pencil, PLM per face
+// (hydro/reconstruct/plm.h:116-153, mhd/reconstruct/plm.h:18-50).
+template <class FT, bool MHD>
+__global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int axis, int step) {
+    if (m.ts->skip) return;
+    constexpr int NV = MHD ? 8 : 5;
+    const int b = blockIdx.y;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nfa[axis]) return;
+    int D[3], o[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        D[a] = m.n[a] + (a == axis ? 1 : 2);
+        o[a] = a == axis ? 0 : 1;
+    }
+    int p[3];
+    p[0] = (int)(t % D[0]) - o[0];
+    p[1] = (int)((t / D[0]) % D[1]) - o[1];
+    p[2] = (int)(t / ((size_t)D[0] * D[1])) - o[2];
+    const int a1 = mod3(axis + 1), a2 = mod3(axis + 2);
+    const bool in1 = p[a1] >= 0 && p[a1] < m.n[a1];
+    const bool in2 = p[a2] >= 0 && p[a2] < m.n[a2];
+    if constexpr (MHD) {
+        if (!in1 && !in2) return;
+    } else {
+        if (!in1 || !in2) return;
+    }
+    FT q[4][NV];
+    const int f = p[axis];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int pc[3] = {p[0], p[1], p[2]};
+        pc[axis] = f - 2 + s;
+        load_cell<FT, NV>(m, b, pc, q[s]);
+    }
+    FT ql[NV], qr[NV], flx[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (MHD && v == BCC + axis) continue;
+        plm<FT>(q[0][v], q[1][v], q[2][v], q[3][v], ql[v], qr[v]);
+    }
+    if constexpr (MHD) {
+        const FT bn = (FT)load_bf(m, b, step, axis, p);
+        ql[BCC + axis] = bn;
+        qr[BCC + axis] = bn;
+    }
+    face_flux<FT, MHD>(axis, ql, qr, flx, (FT)m.gamma);
+    FT *F = m.F[axis] + (size_t)b * NV * m.nfa[axis] + t;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[(size_t)v * m.nfa[axis]] = flx[v];
+}
+
+// ------------------------------------------------------------------- EMF
+template <class FT>
+__device__ __forceinline__ FT ec_at(const MeshView<FT> &m, int b, const int *p, const int *ax) {
+    FT w[8];
+    load_cell<FT, 8>(m, b, p, w);
+    return w[BCC + ax[1]] * w[MOM + ax[2]] - w[BCC + ax[2]] * w[MOM + ax[1]];
+}
+
+template <class FT>
+__device__ __forceinline__ FT flux_at(const MeshView<FT> &m, int b, int a, const int *p, int v) {
+    return m.F[a][((size_t)b * 8 + v) * m.nfa[a] + xidx(m.n, a, p)];
+}
+
+// Edge EMF with the corner-upwind correction, mhd/integrator/integrator.h
+// :189-327 (3-D, w_e = 0.25).  Cell-centred E outside the block is B x v
+// from the neighbour's primitives (:201-213); inside it equals the c2p ec.
+template <class FT>
+__global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
+    if (m.ts->skip) return;
+    const int b = blockIdx.y;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nface) return;
+    int i[3];
+    i[0] = (int)(t % (m.n[0] + 1));
+    i[1] = (int)((t / (m.n[0] + 1)) % (m.n[1] + 1));
+    i[2] = (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)));
+    for (int ax = 0; ax < 3; ++ax) {
+        if (i[ax] >= m.n[ax]) continue;
+        const int ax_1 = mod3(1 + ax), ax_2 = mod3(2 + ax);
+        const int axis[3] = {ax, ax_1, ax_2};
+        FT ec[2][2];
+        ec[1][1] = ec_at(m, b, i, axis);
+        --i[ax_1];
+        ec[1][0] = ec_at(m, b, i, axis);
+        --i[ax_2];
+        ec[0][0] = ec_at(m, b, i, axis);
+        ++i[ax_1];
+        ec[0][1] = ec_at(m, b, i, axis);
+        ++i[ax_2];
+        FT ee = 0;
+        const FT w_e = (FT)0.25f;
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi) {
+            const int a_para = pi == 0 ? ax_1 : ax_2;
+            const int a_perp = pi == 0 ? ax_2 : ax_1;
+            --i[a_perp];
+            const FT ef_m = flux_at(m, b, a_para, i, BCC + ax);
+            ++i[a_perp];
+            const FT ef_p = flux_at(m, b, a_para, i, BCC + ax);
+            ee += w_e * (ef_m + ef_p);
+            const FT frho_p = flux_at(m, b, a_perp, i, RHO);
+            --i[a_para];
+            const FT frho_m = flux_at(m, b, a_perp, i, RHO);
+            ++i[a_para];
+            const FT de_p = (frho_p > 0 ? (ec[0][1] - ef_m) : (ec[1][1] - ef_p));
+            const FT de_m = (frho_m > 0 ? -(ec[0][0] - ef_m) : -(ec[1][0] - ef_p));
+            ee += w_e * (de_m - de_p);
+            if (pi == 0) {
+                const FT swp = ec[0][1];
+                ec[0][1] = ec[1][0];
+                ec[1][0] = swp;
+            }
+        }
+        m.EE[((size_t)b * 3 + ax) * m.nface + t] = ee;
+    }
+}
+
+// ---------------------------------------------------------------- update
+// Flux divergence + Heun slot (hydro/integrator/integrator.h:56-68, 156-182,
+// 241-265; rk2.h:20-41) and, for MHD, the CT update of the face field
+// (mhd/integrator/integrator.h:385-462 with rk2.h:191-205).
+template <class FT, bool MHD>
+__global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
+    if (m.ts->skip) return;
+    constexpr int NV = MHD ? 8 : 5;
+    const int b = blockIdx.y;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nitem = MHD ? m.nface : m.ncell;
+    if (t >= nitem) return;
+    int i[3];
+    const int e0 = m.n[0] + (MHD ? 1 : 0), e1 = m.n[1] + (MHD ? 1 : 0);
+    i[0] = (int)(t % e0);
+    i[1] = (int)((t / e0) % e1);
+    i[2] = (int)(t / ((size_t)e0 * e1));
+    const double dt = m.ts->dt_use * (step == 0 ? 1.0 : 0.5);
+    const BlockGeo &g = m.geo[b];
+    if (i[0] < m.n[0] && i[1] < m.n[1] && i[2] < m.n[2]) {
+        const size_t c = cidx(m.n, i[0], i[1], i[2]);
+        const double *Us = m.U + ((size_t)b * 2 + step) * 5 * m.ncell + c;
+        double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * m.ncell + c;
+        double uu[5];
+        if (step == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) uu[v] = Us[(size_t)v * m.ncell];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                uu[v] = (Us[(size_t)v * m.ncell] + Ut[(size_t)v * m.ncell]) * 0.5f;
+        }
+        const double dt_dv = dt / g.vol;
+        for (int ax = 0; ax < m.ndim; ++ax) {
+            int p1[3] = {i[0], i[1], i[2]};
+            ++p1[ax];
+            const size_t x0 = xidx(m.n, ax, i), x1 = xidx(m.n, ax, p1);
+            const FT *F = m.F[ax] + (size_t)b * NV * m.nfa[ax];
+            if constexpr (std::is_same_v<FT, double>) {
+                const double fa = g.sfc[ax] * dt_dv;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    double res(0);
+                    if (fa > 0) res += fa * F[(size_t)v * m.nfa[ax] + x0];
+                    if (fa > 0) res -= fa * F[(size_t)v * m.nfa[ax] + x1];
+                    uu[v] += res;
+                }
+            } else {
+                const float fa = (float)(g.sfc[ax] * dt_dv);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    double res(0);
+                    if (fa > 0) res += (double)(fa * F[(size_t)v * m.nfa[ax] + x0]);
+                    if (fa > 0) res -= (double)(fa * F[(size_t)v * m.nfa[ax] + x1]);
+                    uu[v] += res;
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ut[(size_t)v * m.ncell] = uu[v];
+    }
+    if constexpr (MHD) {
+        const FT *EE = m.EE + (size_t)b * 3 * m.nface;
+        double *Bs = m.BF + ((size_t)b * 2 + step) * 3 * m.nface + t;
+        double *Bt = m.BF + ((size_t)b * 2 + (1 - step)) * 3 * m.nface + t;
+        const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int ax_1 = mod3(a + 1), ax_2 = mod3(a + 2);
+            if (i[ax_1] >= m.n[ax_1] || i[ax_2] >= m.n[ax_2]) continue;
+            // FP64 edge circulation (also in the mixed form: the EMF is
+            // widened before the product so CT keeps div B at round-off).
+            double db = (double)EE[(size_t)ax_1 * m.nface + t] * g.dx[ax_1];
+            db -= (double)EE[(size_t)ax_2 * m.nface + t] * g.dx[ax_2];
+            db += (double)EE[(size_t)ax_2 * m.nface + t + sh[ax_1]] * g.dx[ax_2];
+            db -= (double)EE[(size_t)ax_1 * m.nface + t + sh[ax_2]] * g.dx[ax_1];
+            const double sfc = g.sfc[a];
+            db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
+            const double bs = Bs[(size_t)a * m.nface];
+            Bt[(size_t)a * m.nface] =
+                db + (step == 0 ? bs : (bs + Bt[(size_t)a * m.nface]) / 2);
+        }
+    }
+}
+
+// Max |div B| per workgroup (diagnostic): sum over axes of the face
+// difference divided by the cell width, slot 0.
+template <class FT>
+__global__ void __launch_bounds__(kThreads) k_divb(MeshView<FT> m, double *part) {
+    const int b = blockIdx.y;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0;
+    if (c < m.ncell) {
+        const int i = (int)(c % m.n[0]);
+        const int j = (int)((c / m.n[0]) % m.n[1]);
+        const int k = (int)(c / ((size_t)m.n[0] * m.n[1]));
+        const double *BF = m.BF + (size_t)b * 2 * 3 * m.nface;
+        const size_t f0 = fidx(m.n, i, j, k);
+        const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
+        double s = 0;
+        for (int a = 0; a < 3; ++a)
+            s += (BF[(size_t)a * m.nface + f0 + sh[a]] - BF[(size_t)a * m.nface + f0]) / m.geo[b].dx[a];
+        v = kfabs(s);
+    }
+    v = -v;
+    const double r = block_min(v);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = -r;
+}
+
+}  // namespace kr

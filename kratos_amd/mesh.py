@@ -1,0 +1,282 @@
+"""Uniform block meshes and the device context.
+
+``MeshSpec`` reproduces the reference's Cartesian block generator for a
+same-level mesh: block order, per-block geometry (meshgen.cpp:286-321,
+``loc(1) - loc(0)`` in the same float arithmetic) and the 26-neighbour table
+with periodic wrap (tree.cpp:77-84, 177-197).  ``Context`` owns one
+``kr_ctx`` of the C-ABI and moves global numpy fields in and out of it
+through the per-block AoS layout of the reference's host arrays.
+
+Global field layout used by the Python side:
+  u  : (5, Nz, Ny, Nx) conserved rho, E, m0, m1, m2
+  bx : (Nz, Ny, Nx+1), by : (Nz, Ny+1, Nx), bz : (Nz+1, Ny, Nx) face fields
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi
+
+
+@dataclass
+class MeshSpec:
+    n_cell_global: tuple
+    n_cell_block: tuple
+    x_min: tuple = (0.0, 0.0, 0.0)
+    x_max: tuple = (1.0, 1.0, 1.0)
+    bnd: tuple = ("per", "per", "per", "per", "per", "per")
+
+    def __post_init__(self):
+        self.n_cell_global = tuple(int(v) for v in self.n_cell_global)
+        self.n_cell_block = tuple(int(v) for v in self.n_cell_block)
+        for a in range(3):
+            if self.n_cell_global[a] % self.n_cell_block[a]:
+                raise ValueError("Mesh size indivisible by sub-mesh.")
+        self.nblk = tuple(self.n_cell_global[a] // self.n_cell_block[a] for a in range(3))
+        self.periodic = tuple(self.bnd[2 * a] == "per" for a in range(3))
+
+    @property
+    def n_blocks(self):
+        return self.nblk[0] * self.nblk[1] * self.nblk[2]
+
+    def bid(self, r):
+        return (r[2] * self.nblk[1] + r[1]) * self.nblk[0] + r[0]
+
+    def reg(self, b):
+        nx, ny, _ = self.nblk
+        return (b % nx, (b // nx) % ny, b // (nx * ny))
+
+    def geometry(self, b):
+        """(xf0, dx0) of block b, as meshgen::set_coord_axes computes them."""
+        r = self.reg(b)
+        xf0, dx0 = [0.0] * 3, [0.0] * 3
+        for a in range(3):
+            n = self.n_cell_block[a]
+
+            def loc(i, a=a, n=n):
+                if n == 1:
+                    return i
+                i = (r[a] + i / n) / float(self.nblk[a])
+                return self.x_min[a] + i * (self.x_max[a] - self.x_min[a])
+
+            xf0[a] = loc(0.0)
+            dx0[a] = loc(1.0) - xf0[a]
+        return xf0, dx0
+
+    def neighbours(self, b):
+        r = self.reg(b)
+        out = [-1] * 27
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    d = (dx, dy, dz)
+                    k = ((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)
+                    q = []
+                    ok = True
+                    for a in range(3):
+                        v = r[a] + d[a]
+                        if self.n_cell_block[a] == 1 and d[a] != 0:
+                            ok = False
+                            break
+                        if v < 0 or v >= self.nblk[a]:
+                            if not self.periodic[a]:
+                                ok = False
+                                break
+                            v %= self.nblk[a]
+                        q.append(v)
+                    out[k] = self.bid(q) if ok else -1
+        return out
+
+    def block_descs(self):
+        arr = (_capi.KrBlockDesc * self.n_blocks)()
+        for b in range(self.n_blocks):
+            d = arr[b]
+            d.id = b
+            d.level = 0
+            r = self.reg(b)
+            xf0, dx0 = self.geometry(b)
+            for a in range(3):
+                d.reg[a] = r[a]
+                d.n_ceff[a] = self.n_cell_block[a]
+                d.xf0[a] = xf0[a]
+                d.dx0[a] = dx0[a]
+            for k, v in enumerate(self.neighbours(b)):
+                d.nbr[k] = v
+        return arr
+
+    # ---------------------------------------------------- block <-> global
+    def block_slices(self, b):
+        r = self.reg(b)
+        n = self.n_cell_block
+        return tuple(slice(r[a] * n[a], (r[a] + 1) * n[a]) for a in range(3))
+
+    def split(self, u, bfs=None):
+        """Global fields -> per-block (u_aos, bf_aos) in the reference layout."""
+        mhd = bfs is not None
+        nvar = 8 if mhd else 5
+        n = self.n_cell_block
+        out = []
+        for b in range(self.n_blocks):
+            sx, sy, sz = self.block_slices(b)
+            ub = u[:, sz, sy, sx]                       # (5, nz, ny, nx)
+            if mhd:
+                bx, by, bz = bfs
+                bcc = np.stack([
+                    0.5 * (bx[sz, sy, sx.start:sx.stop + 1][:, :, :-1] + bx[sz, sy, sx.start:sx.stop + 1][:, :, 1:]),
+                    0.5 * (by[sz, sy.start:sy.stop + 1, sx][:, :-1, :] + by[sz, sy.start:sy.stop + 1, sx][:, 1:, :]),
+                    0.5 * (bz[sz.start:sz.stop + 1, sy, sx][:-1] + bz[sz.start:sz.stop + 1, sy, sx][1:]),
+                ])
+                ub = np.concatenate([ub, bcc])
+            u_aos = np.ascontiguousarray(np.moveaxis(ub, 0, -1)).reshape(-1)
+            bf_aos = None
+            if mhd:
+                bf_aos = np.zeros((n[2] + 5, n[1] + 5, n[0] + 5, 3))
+                r = self.reg(b)
+                for c, g in enumerate(bfs):
+                    zi = (np.arange(n[2] + 1) + r[2] * n[2])
+                    yi = (np.arange(n[1] + 1) + r[1] * n[1])
+                    xi = (np.arange(n[0] + 1) + r[0] * n[0])
+                    # transverse wrap (periodic); the component axis keeps N+1 faces
+                    if c != 2:
+                        zi = zi % self.n_cell_global[2]
+                    if c != 1:
+                        yi = yi % self.n_cell_global[1]
+                    if c != 0:
+                        xi = xi % self.n_cell_global[0]
+                    bf_aos[2:n[2] + 3, 2:n[1] + 3, 2:n[0] + 3, c] = g[np.ix_(zi, yi, xi)]
+                bf_aos = bf_aos.reshape(-1)
+            out.append((u_aos, bf_aos))
+        return out
+
+    def gather(self, blocks, mhd):
+        """Per-block (u_aos, bf_aos) -> global (u, (bx, by, bz))."""
+        nvar = 8 if mhd else 5
+        N = self.n_cell_global
+        n = self.n_cell_block
+        u = np.zeros((5, N[2], N[1], N[0]))
+        bfs = None
+        if mhd:
+            bfs = (np.zeros((N[2], N[1], N[0] + 1)), np.zeros((N[2], N[1] + 1, N[0])),
+                   np.zeros((N[2] + 1, N[1], N[0])))
+        for b, (u_aos, bf_aos) in enumerate(blocks):
+            sx, sy, sz = self.block_slices(b)
+            ub = np.moveaxis(np.asarray(u_aos).reshape(n[2], n[1], n[0], nvar), -1, 0)
+            u[:, sz, sy, sx] = ub[:5]
+            if mhd:
+                f = np.asarray(bf_aos).reshape(n[2] + 5, n[1] + 5, n[0] + 5, 3)
+                bfs[0][sz, sy, sx.start:sx.stop + 1] = f[2:n[2] + 2, 2:n[1] + 2, 2:n[0] + 3, 0]
+                bfs[1][sz, sy.start:sy.stop + 1, sx] = f[2:n[2] + 2, 2:n[1] + 3, 2:n[0] + 2, 1]
+                bfs[2][sz.start:sz.stop + 1, sy, sx] = f[2:n[2] + 3, 2:n[1] + 2, 2:n[0] + 2, 2]
+        return u, bfs
+
+
+@dataclass
+class Physics:
+    mhd: bool = True
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.3
+    rho_floor: float = 1e-9
+    tmp_floor: float = 1e-9
+    t_0: float = 0.0
+    t_lim: float = 3.4028234663852886e38
+    dt_init: float = 0.0
+    dt_expand: float = 2.0
+    top_up: bool = False
+
+
+class Context:
+    """One device context of the C-ABI over a uniform block mesh."""
+
+    def __init__(self, spec: MeshSpec, phys: Physics, precision="fp64"):
+        lib = _capi.load()
+        self.lib, self.spec, self.phys = lib, spec, phys
+        self.mhd = phys.mhd
+        cfg = _capi.KrConfig()
+        cfg.model = _capi.KR_MHD if phys.mhd else _capi.KR_HYDRO
+        cfg.precision = {"fp64": _capi.KR_FP64, "mixed": _capi.KR_MIXED}[precision]
+        cfg.gamma, cfg.cfl = phys.gamma, phys.cfl
+        cfg.rho_floor, cfg.tmp_floor = phys.rho_floor, phys.tmp_floor
+        for s in range(6):
+            cfg.bnd[s] = _capi.BND_CODES[spec.bnd[s]]
+        cfg.t_0, cfg.t_lim = phys.t_0, phys.t_lim
+        cfg.dt_init, cfg.dt_expand = phys.dt_init, phys.dt_expand
+        cfg.top_up = int(phys.top_up)
+        for a in range(3):
+            cfg.n_cell_block[a] = spec.n_cell_block[a]
+        h = C.c_void_p()
+        rc = lib.kr_create(C.byref(cfg), C.byref(h))
+        _capi.check(rc, None, "kr_create")
+        self.h = h
+        descs = spec.block_descs()
+        _capi.check(lib.kr_add_blocks(h, spec.n_blocks, descs), h, "kr_add_blocks")
+
+    def close(self):
+        if self.h:
+            self.lib.kr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, u, bfs=None):
+        for b, (ua, ba) in enumerate(self.spec.split(u, bfs if self.mhd else None)):
+            ua = np.ascontiguousarray(ua, dtype=np.float64)
+            bp = None
+            if ba is not None:
+                ba = np.ascontiguousarray(ba, dtype=np.float64)
+                bp = ba.ctypes.data
+            _capi.check(self.lib.kr_upload(self.h, b, ua.ctypes.data, bp), self.h, "kr_upload")
+
+    def download(self):
+        n = self.spec.n_cell_block
+        nvar = 8 if self.mhd else 5
+        blocks = []
+        for b in range(self.spec.n_blocks):
+            ua = np.zeros(n[0] * n[1] * n[2] * nvar)
+            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            _capi.check(self.lib.kr_download(self.h, b, ua.ctypes.data,
+                                             ba.ctypes.data if ba is not None else None),
+                        self.h, "kr_download")
+            blocks.append((ua, ba))
+        return self.spec.gather(blocks, self.mhd)
+
+    def step(self):
+        dt = C.c_double()
+        _capi.check(self.lib.kr_step(self.h, C.byref(dt)), self.h, "kr_step")
+        return dt.value
+
+    def advance(self, n_cycles):
+        done = C.c_int()
+        _capi.check(self.lib.kr_advance(self.h, int(n_cycles), C.byref(done)), self.h,
+                    "kr_advance")
+        return done.value
+
+    def time(self):
+        t, dt, seed = C.c_double(), C.c_double(), C.c_double()
+        _capi.check(self.lib.kr_get_time(self.h, C.byref(t), C.byref(dt), C.byref(seed)),
+                    self.h, "kr_get_time")
+        return t.value, dt.value, seed.value
+
+    def divb_max(self):
+        v = C.c_double()
+        _capi.check(self.lib.kr_divb_max(self.h, C.byref(v)), self.h, "kr_divb_max")
+        return v.value
+
+    def profile(self, n_cycles):
+        _capi.check(self.lib.kr_profile(self.h, int(n_cycles)), self.h, "kr_profile")
+        return self.stats()
+
+    def stats(self):
+        s = _capi.KrStats()
+        _capi.check(self.lib.kr_get_stats(self.h, C.byref(s)), self.h, "kr_get_stats")
+        return {
+            "gpu_ms": s.gpu_ms, "cycles": s.cycles, "n_blocks": s.n_blocks,
+            "n_cells": s.n_cells,
+            "kernel_ms": {k: s.kernel_ms[i] for i, k in enumerate(_capi.KERNEL_KINDS)},
+            "kernel_calls": {k: s.kernel_calls[i] for i, k in enumerate(_capi.KERNEL_KINDS)},
+            "alg_bytes_per_cell_update": s.alg_bytes_per_cell_update,
+        }

@@ -1,0 +1,158 @@
+"""Initial-condition generators for the benchmark / parity workloads.
+
+All generators are deterministic and return global fields in the layout of
+``kratos_amd.mesh`` (u: (5, Nz, Ny, Nx); face fields bx, by, bz).  The
+random numbers come from SplitMix64 (Steele, Lea & Flood 2014) so that the
+same seed gives the same state on every machine.
+
+* ``sod``        -- Sod shock tube (config 1: 512x2x1, gamma 1.4).
+* ``turbulence`` -- seeded Fourier-mode velocity plus a face-centred curl of
+  a seeded edge vector potential (div B = 0 to round-off), rescaled to the
+  requested rms Mach number and plasma beta (configs 2, 3, 5).
+* ``blast``      -- MHD blast wave in a uniform oblique field (config 4).
+"""
+import math
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = int(seed) & _MASK
+
+    def next_u64(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & _MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        """Uniform double in [0, 1) from the top 53 bits."""
+        return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def _modes(rng, n_modes, k_max):
+    """n_modes (k, amplitude vector, phase) with k in Z^3, 0 < |k| <= k_max."""
+    out = []
+    span = 2 * k_max + 1
+    while len(out) < n_modes:
+        k = [int(rng.uniform() * span) - k_max for _ in range(3)]
+        k2 = k[0] ** 2 + k[1] ** 2 + k[2] ** 2
+        if k2 == 0 or k2 > k_max * k_max:
+            continue
+        amp = [2.0 * rng.uniform() - 1.0 for _ in range(3)]
+        phase = 2.0 * math.pi * rng.uniform()
+        out.append((k, amp, phase))
+    return out
+
+
+def _field(modes, comp, xs, ys, zs):
+    """sum_m amp_m[comp] * sin(2 pi k_m . x + phase_m) on the grid xs x ys x zs."""
+    f = np.zeros((len(zs), len(ys), len(xs)))
+    for k, amp, ph in modes:
+        if amp[comp] == 0.0:
+            continue
+        ex = np.exp(2j * np.pi * k[0] * xs)
+        ey = np.exp(2j * np.pi * k[1] * ys)
+        ez = np.exp(2j * np.pi * k[2] * zs) * np.exp(1j * ph)
+        f += amp[comp] * np.imag(ez[:, None, None] * ey[None, :, None] * ex[None, None, :])
+    return f
+
+
+def cell_centred_b(bx, by, bz):
+    return (0.5 * (bx[:, :, :-1] + bx[:, :, 1:]), 0.5 * (by[:, :-1, :] + by[:, 1:, :]),
+            0.5 * (bz[:-1] + bz[1:]))
+
+
+def total_energy(rho, pre, vel, bcc, gamma):
+    """E = p/(gamma-1) + rho v^2/2 + |Bcc|^2/2 with the c2p's Bcc convention."""
+    e = pre / (gamma - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    if bcc is not None:
+        e = e + 0.5 * (bcc[0] ** 2 + bcc[1] ** 2 + bcc[2] ** 2)
+    return e
+
+
+def turbulence(N, seed=0, gamma=5.0 / 3.0, mach=0.5, beta=1.0, n_modes=32, k_max=4,
+               beta_range=None):
+    """Seeded MHD turbulence in the periodic unit box with N = (Nx, Ny, Nz) cells.
+
+    rho = 1, p = 1/gamma (c_s = 1).  Velocity: n_modes Fourier modes with
+    0 < |k| <= k_max, rescaled to rms Mach ``mach``.  B: face-centred discrete
+    curl of an edge vector potential built the same way (each component its
+    own n_modes), rescaled to plasma beta = 2 p / <|Bcc|^2>.  With
+    ``beta_range=(lo, hi)`` beta is drawn uniformly from the same stream
+    after the modes.  Returns (u, (bx, by, bz), info).
+    """
+    Nx, Ny, Nz = N
+    rng = SplitMix64(seed)
+    vmodes = _modes(rng, n_modes, k_max)
+    amodes = [_modes(rng, n_modes, k_max) for _ in range(3)]
+    if beta_range is not None:
+        beta = beta_range[0] + (beta_range[1] - beta_range[0]) * rng.uniform()
+    xc = (np.arange(Nx) + 0.5) / Nx
+    yc = (np.arange(Ny) + 0.5) / Ny
+    zc = (np.arange(Nz) + 0.5) / Nz
+    xf = np.arange(Nx + 1) / Nx
+    yf = np.arange(Ny + 1) / Ny
+    zf = np.arange(Nz + 1) / Nz
+    rho = np.ones((Nz, Ny, Nx))
+    pre = np.full((Nz, Ny, Nx), 1.0 / gamma)
+    vel = [_field(vmodes, c, xc, yc, zc) for c in range(3)]
+    vrms = math.sqrt(float(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)))
+    cs = math.sqrt(gamma * (1.0 / gamma) / 1.0)
+    s = mach * cs / vrms
+    vel = [v * s for v in vel]
+    # edge potential: A_x at (xc, yf, zf), A_y at (xf, yc, zf), A_z at (xf, yf, zc)
+    ax = _field(amodes[0], 0, xc, yf, zf)
+    ay = _field(amodes[1], 1, xf, yc, zf)
+    az = _field(amodes[2], 2, xf, yf, zc)
+    dx, dy, dz = 1.0 / Nx, 1.0 / Ny, 1.0 / Nz
+    bx = (az[:, 1:, :] - az[:, :-1, :]) / dy - (ay[1:, :, :] - ay[:-1, :, :]) / dz
+    by = (ax[1:, :, :] - ax[:-1, :, :]) / dz - (az[:, :, 1:] - az[:, :, :-1]) / dx
+    bz = (ay[:, :, 1:] - ay[:, :, :-1]) / dx - (ax[:, 1:, :] - ax[:, :-1, :]) / dy
+    bcc = cell_centred_b(bx, by, bz)
+    b2 = float(np.mean(bcc[0] ** 2 + bcc[1] ** 2 + bcc[2] ** 2))
+    sb = math.sqrt(2.0 * (1.0 / gamma) / beta / b2)
+    bx, by, bz = bx * sb, by * sb, bz * sb
+    bcc = cell_centred_b(bx, by, bz)
+    u = np.empty((5, Nz, Ny, Nx))
+    u[0] = rho
+    u[1] = total_energy(rho, pre, vel, bcc, gamma)
+    for c in range(3):
+        u[2 + c] = rho * vel[c]
+    return u, (bx, by, bz), {"beta": beta, "mach": mach, "seed": seed}
+
+
+def sod(N=(512, 2, 1), gamma=1.4, x_split=0.5, left=(1.0, 1.0), right=(0.125, 0.1),
+        x_min=0.0, x_max=1.0):
+    """Sod tube along x: (rho, p) = left for x < x_split, right otherwise."""
+    Nx, Ny, Nz = N
+    dx = (x_max - x_min) / Nx
+    xc = x_min + (np.arange(Nx) + 0.5) * dx
+    rho = np.where(xc < x_split, left[0], right[0])
+    pre = np.where(xc < x_split, left[1], right[1])
+    u = np.zeros((5, Nz, Ny, Nx))
+    u[0] = rho[None, None, :]
+    u[1] = (pre / (gamma - 1.0))[None, None, :]
+    return u
+
+
+def blast(N, gamma=5.0 / 3.0, rho0=1.0, p0=0.1, p_in=10.0, r_in=0.1,
+          b0=(1.0 / math.sqrt(2.0), 1.0 / math.sqrt(2.0), 0.0),
+          x_min=(-0.5, -0.5, -0.5), x_max=(0.5, 0.5, 0.5)):
+    """MHD blast: uniform rho, p, oblique B; p = p_in inside radius r_in."""
+    Nx, Ny, Nz = N
+    xs = [x_min[a] + (np.arange(N[a]) + 0.5) * (x_max[a] - x_min[a]) / N[a] for a in range(3)]
+    r2 = (xs[2][:, None, None] ** 2 + xs[1][None, :, None] ** 2 + xs[0][None, None, :] ** 2)
+    pre = np.where(r2 < r_in * r_in, p_in, p0)
+    rho = np.full((Nz, Ny, Nx), rho0)
+    bx = np.full((Nz, Ny, Nx + 1), b0[0])
+    by = np.full((Nz, Ny + 1, Nx), b0[1])
+    bz = np.full((Nz + 1, Ny, Nx), b0[2])
+    u = np.zeros((5, Nz, Ny, Nx))
+    u[0] = rho
+    u[1] = total_energy(rho, pre, [0 * rho] * 3, cell_centred_b(bx, by, bz), gamma)
+    return u, (bx, by, bz)
