@@ -282,11 +282,20 @@ struct Launcher {
                 hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1024), 0, c->stream, c->ts,
                                    (const double *)c->dt_part, (int)c->n_dt_part);
             });
-        for (int ax = c->ndim - 1; ax >= 0; --ax)
-            run(KR_K_FLUX_X + ax, [&] {
-                hipLaunchKernelGGL((k_flux<FT, MHD>), dim3(ceil_div(c->nfa[ax], kThreads), nb), blk,
-                                   0, c->stream, m, ax, step);
+        if (c->ndim > 2)
+            run(KR_K_FLUX_Z, [&] {
+                hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), nb),
+                                   blk, 0, c->stream, m, step);
             });
+        if (c->ndim > 1)
+            run(KR_K_FLUX_Y, [&] {
+                hipLaunchKernelGGL((k_flux<FT, MHD, 1>), dim3(ceil_div(c->nfa[1], kThreads), nb),
+                                   blk, 0, c->stream, m, step);
+            });
+        run(KR_K_FLUX_X, [&] {
+            hipLaunchKernelGGL((k_flux<FT, MHD, 0>), dim3(ceil_div(c->nfa[0], kThreads), nb), blk,
+                               0, c->stream, m, step);
+        });
         if constexpr (MHD)
             run(KR_K_EMF, [&] {
                 hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), nb), blk, 0,

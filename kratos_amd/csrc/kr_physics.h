@@ -410,22 +410,33 @@ __device__ __forceinline__ void hlld(FT *flx, const FT *wli, const FT *wri, FT g
 // Face flux in the mesh frame for a sweep along `axis`: hllc.h:32-43 and
 // hlld.h:29-47 (the B-flux slots are rewritten as edge EMFs, normal slot 0).
 // ql / qr are rotated in place.
-template <class FT, bool MHD>
-__device__ __forceinline__ void face_flux(int axis, FT *ql, FT *qr, FT *flx, FT gam) {
-    idx_swap(ql + MOM, axis);
-    idx_swap(qr + MOM, axis);
+// Rotation with a compile-time axis, so that every register array index is
+// static (no scratch spills).
+template <int AXIS, class T> __device__ __forceinline__ void rot(T *x) {
+    if constexpr (AXIS % 3 != 0) {
+        const T s0 = x[AXIS % 3], s1 = x[(AXIS + 1) % 3], s2 = x[(AXIS + 2) % 3];
+        x[0] = s0;
+        x[1] = s1;
+        x[2] = s2;
+    }
+}
+
+template <class FT, bool MHD, int AXIS>
+__device__ __forceinline__ void face_flux(FT *ql, FT *qr, FT *flx, FT gam) {
+    rot<AXIS>(ql + MOM);
+    rot<AXIS>(qr + MOM);
     if constexpr (MHD) {
-        idx_swap(ql + BCC, axis);
-        idx_swap(qr + BCC, axis);
+        rot<AXIS>(ql + BCC);
+        rot<AXIS>(qr + BCC);
         hlld<FT>(flx, ql, qr, gam);
-        idx_swap(flx + MOM, 3 - axis);
+        rot<3 - AXIS>(flx + MOM);
         const FT e0 = flx[BCC + 2], e1 = -flx[BCC + 1];
-        flx[BCC + mod3(axis + 1)] = e0;
-        flx[BCC + mod3(axis + 2)] = e1;
-        flx[BCC + axis] = 0;
+        flx[BCC + (AXIS + 1) % 3] = e0;
+        flx[BCC + (AXIS + 2) % 3] = e1;
+        flx[BCC + AXIS] = 0;
     } else {
         hllc<FT>(flx, ql, qr, gam);
-        idx_swap(flx + MOM, 3 - axis);
+        rot<3 - AXIS>(flx + MOM);
     }
 }
 

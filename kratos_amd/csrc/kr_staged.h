@@ -51,28 +51,31 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
         out |= d[a] != 0;
     }
     int blk = b;
-    int flip = -1;
+    int flip = -1;      // reflect: negate v_ax;  outflow: clamp inflow of v_ax
+    int clamp = -1;     // 2*ax + side of an outflow side
     if (out) {
         const int nb = m.geo[b].nbr[nbr_dir(m, d)];
         if (nb >= 0) {
             blk = nb;
 #pragma unroll
             for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
-        } else {
-            // physical side: one axis out of range (hydro face ghosts)
-            int ax = 0;
+        } else if constexpr (NV == 5) {
+            // physical side: one axis out of range (hydro face ghosts);
+            // every index below is resolved with unrolled selects so the
+            // register arrays keep static indices.
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (d[a] != 0) ax = a;
-            const int side = d[ax] > 0;
-            const int kind = m.bnd[2 * ax + side];
-            const int g = side ? p[ax] - m.n[ax] : -1 - p[ax];
-            if (kind == 3) {                       // reflect
-                p[ax] = side ? m.n[ax] - 1 - g : g;
-                flip = ax;
-            } else {                               // outflow / free
-                p[ax] = side ? m.n[ax] - 1 : 0;
-                if (kind == 1) flip = 3 + 2 * ax + side;
+            for (int a = 0; a < 3; ++a) {
+                if (d[a] == 0) continue;
+                const int side = d[a] > 0;
+                const int kind = m.bnd[2 * a + side];
+                const int g = side ? p[a] - m.n[a] : -1 - p[a];
+                if (kind == 3) {                       // reflect
+                    p[a] = side ? m.n[a] - 1 - g : g;
+                    flip = a;
+                } else {                               // outflow / free
+                    p[a] = side ? m.n[a] - 1 : 0;
+                    if (kind == 1) clamp = 2 * a + side;
+                }
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) p[a] = p[a] < 0 ? 0 : (p[a] >= m.n[a] ? m.n[a] - 1 : p[a]);
@@ -81,11 +84,15 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
     const FT *src = m.W + (size_t)blk * NV * m.ncell + cidx(m.n, p[0], p[1], p[2]);
 #pragma unroll
     for (int v = 0; v < NV; ++v) w[v] = src[(size_t)v * m.ncell];
-    if (flip >= 0 && flip < 3) {
-        w[MOM + flip] = w[MOM + flip] * -1;
-    } else if (flip >= 3) {                        // outflow inflow clamp
-        const int ax = (flip - 3) >> 1, side = (flip - 3) & 1;
-        if (w[MOM + ax] * (side ? -1 : 1) > 0) w[MOM + ax] = 0;
+    if constexpr (NV == 5) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (flip == a) w[MOM + a] = w[MOM + a] * -1;
+            if ((clamp >> 1) == a && clamp >= 0) {
+                const int side = clamp & 1;
+                if (w[MOM + a] * (side ? -1 : 1) > 0) w[MOM + a] = 0;
+            }
+        }
     }
 }
 
@@ -205,8 +212,9 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
 // :334-352 (hydro) or mhd/integrator/data.h:53-141 (MHD lines, including the
 // transverse ghost lines) for the pencil, PLM per face
 // (hydro/reconstruct/plm.h:116-153, mhd/reconstruct/plm.h:18-50).
-template <class FT, bool MHD>
-__global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int axis, int step) {
+template <class FT, bool MHD, int AXIS>
+__global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
+    constexpr int axis = AXIS;
     if (m.ts->skip) return;
     constexpr int NV = MHD ? 8 : 5;
     const int b = blockIdx.y;
@@ -222,7 +230,7 @@ __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int axis, int
     p[0] = (int)(t % D[0]) - o[0];
     p[1] = (int)((t / D[0]) % D[1]) - o[1];
     p[2] = (int)(t / ((size_t)D[0] * D[1])) - o[2];
-    const int a1 = mod3(axis + 1), a2 = mod3(axis + 2);
+    constexpr int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
     const bool in1 = p[a1] >= 0 && p[a1] < m.n[a1];
     const bool in2 = p[a2] >= 0 && p[a2] < m.n[a2];
     if constexpr (MHD) {
@@ -249,7 +257,7 @@ __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int axis, int
         ql[BCC + axis] = bn;
         qr[BCC + axis] = bn;
     }
-    face_flux<FT, MHD>(axis, ql, qr, flx, (FT)m.gamma);
+    face_flux<FT, MHD, AXIS>(ql, qr, flx, (FT)m.gamma);
     FT *F = m.F[axis] + (size_t)b * NV * m.nfa[axis] + t;
 #pragma unroll
     for (int v = 0; v < NV; ++v) F[(size_t)v * m.nfa[axis]] = flx[v];

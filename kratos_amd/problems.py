@@ -49,17 +49,40 @@ def _modes(rng, n_modes, k_max):
     return out
 
 
-def _field(modes, comp, xs, ys, zs):
-    """sum_m amp_m[comp] * sin(2 pi k_m . x + phase_m) on the grid xs x ys x zs."""
-    f = np.zeros((len(zs), len(ys), len(xs)))
+def _fields(modes, comps, xs, ys, zs):
+    """sum_m amp_m[c] * sin(2 pi k_m . x + phase_m) on the grid xs x ys x zs for
+    each c in comps, evaluated separably: sin(t_yz + t_x) = S_yz cos t_x + C_yz sin t_x.
+    Uses torch (multi-threaded, CPU) when importable, numpy otherwise."""
+    try:
+        import torch
+        xp, T = torch, True
+        to = lambda a: torch.from_numpy(np.ascontiguousarray(a))
+    except Exception:  # pragma: no cover - numpy fallback for IC generation only
+        xp, T = np, False
+        to = lambda a: a
+    xs, ys, zs = to(np.asarray(xs, dtype=np.float64)), to(np.asarray(ys, dtype=np.float64)), \
+        to(np.asarray(zs, dtype=np.float64))
+    shape = (len(zs), len(ys), len(xs))
+    out = [xp.zeros(shape, dtype=xp.float64) for _ in comps]
+    two_pi = 2.0 * math.pi
     for k, amp, ph in modes:
-        if amp[comp] == 0.0:
-            continue
-        ex = np.exp(2j * np.pi * k[0] * xs)
-        ey = np.exp(2j * np.pi * k[1] * ys)
-        ez = np.exp(2j * np.pi * k[2] * zs) * np.exp(1j * ph)
-        f += amp[comp] * np.imag(ez[:, None, None] * ey[None, :, None] * ex[None, None, :])
-    return f
+        tx = two_pi * k[0] * xs
+        tyz = two_pi * (k[1] * ys[None, :] + k[2] * zs[:, None]) + ph
+        cx, sx = xp.cos(tx), xp.sin(tx)
+        cyz, syz = xp.cos(tyz), xp.sin(tyz)
+        s3 = syz[:, :, None] * cx[None, None, :]
+        if T:
+            s3.addcmul_(cyz[:, :, None], sx[None, None, :])
+        else:
+            s3 += cyz[:, :, None] * sx[None, None, :]
+        for o, c in zip(out, comps):
+            if amp[c] == 0.0:
+                continue
+            if T:
+                o.add_(s3, alpha=amp[c])
+            else:
+                o += amp[c] * s3
+    return [o.numpy() if T else o for o in out]
 
 
 def cell_centred_b(bx, by, bz):
@@ -100,15 +123,15 @@ def turbulence(N, seed=0, gamma=5.0 / 3.0, mach=0.5, beta=1.0, n_modes=32, k_max
     zf = np.arange(Nz + 1) / Nz
     rho = np.ones((Nz, Ny, Nx))
     pre = np.full((Nz, Ny, Nx), 1.0 / gamma)
-    vel = [_field(vmodes, c, xc, yc, zc) for c in range(3)]
+    vel = _fields(vmodes, (0, 1, 2), xc, yc, zc)
     vrms = math.sqrt(float(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)))
     cs = math.sqrt(gamma * (1.0 / gamma) / 1.0)
     s = mach * cs / vrms
     vel = [v * s for v in vel]
     # edge potential: A_x at (xc, yf, zf), A_y at (xf, yc, zf), A_z at (xf, yf, zc)
-    ax = _field(amodes[0], 0, xc, yf, zf)
-    ay = _field(amodes[1], 1, xf, yc, zf)
-    az = _field(amodes[2], 2, xf, yf, zc)
+    ax = _fields(amodes[0], (0,), xc, yf, zf)[0]
+    ay = _fields(amodes[1], (1,), xf, yc, zf)[0]
+    az = _fields(amodes[2], (2,), xf, yf, zc)[0]
     dx, dy, dz = 1.0 / Nx, 1.0 / Ny, 1.0 / Nz
     bx = (az[:, 1:, :] - az[:, :-1, :]) / dy - (ay[1:, :, :] - ay[:-1, :, :]) / dz
     by = (ax[1:, :, :] - ax[:-1, :, :]) / dz - (az[:, :, 1:] - az[:, :, :-1]) / dx

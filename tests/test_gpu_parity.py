@@ -60,3 +60,41 @@ def test_mhd_turbulence_fp64(gpu_lib, blk):
         assert rel_l1(gu[v], ou[v]) <= 1e-12, v
     for a in range(3):
         assert rel_l1(gb[a], ob[a]) <= 1e-12, a
+
+
+@pytest.mark.parametrize("beta", [1.0, 0.01])
+def test_mhd_turbulence_mixed(gpu_lib, beta):
+    """Mixed form (FP32 reconstruction/Riemann/EMF, FP64 state, CT, c2p, dt)
+    against the FP64 oracle: relative L1 of rho, E and B within 1e-5."""
+    N = (32, 32, 32)
+    spec = kratos_amd.MeshSpec(N, (16, 16, 16))
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    u, b, _ = problems.turbulence(N, seed=11, beta=beta)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, "mixed")
+    assert abs(gt[0] - ot[0]) <= 1e-5 * ot[0]
+    assert rel_l1(gu[0], ou[0]) <= 1e-5
+    assert rel_l1(gu[1], ou[1]) <= 1e-5
+    for a in range(3):
+        assert rel_l1(gb[a], ob[a]) <= 1e-5, a
+
+
+def test_mhd_single_block_self_periodic(gpu_lib):
+    """One 16^3 block that is its own 26 neighbours (the config-5 box shape)."""
+    N = (16, 16, 16)
+    spec = kratos_amd.MeshSpec(N, N)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    u, b, _ = problems.turbulence(N, seed=5, beta=0.3)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 12)
+    np.testing.assert_array_equal(gu, ou)
+    for a in range(3):
+        np.testing.assert_array_equal(gb[a], ob[a])
+
+
+def test_hydro_3d_blocks_reflect(gpu_lib):
+    """3-D hydro with reflecting / free / outflow sides over 2x2x2 blocks."""
+    N = (24, 16, 16)
+    spec = kratos_amd.MeshSpec(N, (12, 8, 8), bnd=("ref", "ref", "fre", "out", "per", "per"))
+    phys = kratos_amd.Physics(mhd=False, gamma=1.4, cfl=0.3)
+    u, _, _ = problems.turbulence(N, seed=2)
+    (gu, _, gt), (ou, _, ot) = run_both(spec, phys, u, None, 8)
+    np.testing.assert_array_equal(gu, ou)
