@@ -60,9 +60,9 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / s) if s > 0 else float(np.abs(a - b).sum())
 
 
-def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles):
+def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="auto"):
     import kratos_amd
-    ctx = kratos_amd.Context(spec, phys, precision)
+    ctx = kratos_amd.Context(spec, phys, precision, path=path)
     ctx.upload(u, b)
     if warmup:
         ctx.advance(warmup)
@@ -114,6 +114,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--block", type=int, default=32)
     ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--path", default="auto", choices=["auto", "staged", "fused"])
     ap.add_argument("--profile-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -135,7 +136,8 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     res, states = {}, {}
     for form in forms:
-        ctx, r = run_form(spec, phys, u, b, form, args.steps, args.warmup, args.profile_cycles)
+        ctx, r = run_form(spec, phys, u, b, form, args.steps, args.warmup, args.profile_cycles,
+                          args.path)
         r["rate"] = r["n_cells"] * r["cycles"] / r["wall"]
         states[form] = ctx.download()
         r["divb"] = ctx.divb_max()

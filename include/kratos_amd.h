@@ -44,7 +44,12 @@ typedef struct kr_config {
     double t_0, t_lim, dt_init, dt_expand;     /* [cycle] keys             */
     int    top_up;         /* [cycle] t_lim_top_up                            */
     int    n_cell_block[3];
+    int    kernel_path;    /* kr_path: KR_PATH_AUTO picks the fused MHD stage
+                              kernel when the block shape allows it         */
+    int    tile[2];        /* fused-kernel column tile (x, y); 0 = autotune  */
 } kr_config;
+
+enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };
 
 typedef struct kr_block_desc {
     int    id;             /* caller's block id                              */
@@ -70,6 +75,8 @@ enum kr_kernel_kind {
     KR_K_C2P = 0, KR_K_DTFIN = 1, KR_K_FLUX_X = 2, KR_K_FLUX_Y = 3,
     KR_K_FLUX_Z = 4, KR_K_EMF = 5, KR_K_UPDATE = 6, KR_K_BEGIN = 7
 };
+/* In the fused path the whole flux / EMF / update / CT stage is one launch,
+ * reported under KR_K_UPDATE.                                             */
 
 typedef struct kr_ctx kr_ctx;
 

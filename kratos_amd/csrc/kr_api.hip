@@ -2,7 +2,7 @@
 // block registry, device pools, AoS <-> SoA packing, launch sequencing and
 // the hipGraph replay of whole cycles.
 #include "../../include/kratos_amd.h"
-#include "kr_staged.h"
+#include "kr_fused.h"
 
 #include <hip/hip_runtime.h>
 
@@ -94,6 +94,8 @@ struct kr_ctx {
     std::unordered_map<int, int> id2idx;
     bool committed = false;
     bool mhd = false, mixed = false;
+    bool fused = false;
+    int tile[2] = {0, 0};
     int n[3] = {0, 0, 0};
     int ndim = 0, nv = 5;
     size_t ncell = 0, nface = 0, nfa[3] = {0, 0, 0};
@@ -242,6 +244,24 @@ void commit(kr_ctx *c) {
     c->n_red_part = c->n_dt_part;
     c->red_part = dalloc<double>(c->n_red_part);
     c->red_out = dalloc<double>(1);
+    // fused stage kernel: 3-D MHD blocks divisible by a supported tile
+    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
+        static const int shapes[][2] = {{32, 8}, {16, 8}, {8, 8}};
+        int tx = c->cfg.tile[0], ty = c->cfg.tile[1];
+        bool ok = false;
+        for (auto &sh : shapes) {
+            if (tx && (sh[0] != tx || sh[1] != ty)) continue;
+            if (c->n[0] % sh[0] == 0 && c->n[1] % sh[1] == 0 && c->n[2] >= 4) {
+                c->tile[0] = sh[0];
+                c->tile[1] = sh[1];
+                ok = true;
+                break;
+            }
+        }
+        if (!ok && c->cfg.kernel_path == KR_PATH_FUSED)
+            throw std::invalid_argument("no fused tile fits the block shape");
+        c->fused = ok;
+    }
     c->stats.n_blocks = (int)nb;
     c->stats.n_cells = (long long)(nb * c->ncell);
     // 5 state passes per Heun cycle of (5 conserved + 3 face-B) FP64 values
@@ -269,6 +289,13 @@ struct Launcher {
         evs.push_back({kind, {a, b}});
     }
 
+    template <class FT, int TX, int TY, int NT> void sweep(const MeshView<FT> &m, int step) {
+        const unsigned nb = (unsigned)c->descs.size();
+        const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
+        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT>), dim3(nt, nb), dim3(NT), 0, c->stream, m,
+                           step);
+    }
+
     template <class FT, bool MHD> void stage(int step) {
         const MeshView<FT> m = c->view<FT>();
         const unsigned nb = (unsigned)c->descs.size();
@@ -282,6 +309,16 @@ struct Launcher {
                 hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1024), 0, c->stream, c->ts,
                                    (const double *)c->dt_part, (int)c->n_dt_part);
             });
+        if constexpr (MHD) {
+            if (c->fused) {
+                run(KR_K_UPDATE, [&] {
+                    if (c->tile[0] == 32) sweep<FT, 32, 8, 512>(m, step);
+                    else if (c->tile[0] == 16) sweep<FT, 16, 8, 256>(m, step);
+                    else sweep<FT, 8, 8, 192>(m, step);
+                });
+                return;
+            }
+        }
         if (c->ndim > 2)
             run(KR_K_FLUX_Z, [&] {
                 hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), nb),

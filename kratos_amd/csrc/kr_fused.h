@@ -1,0 +1,491 @@
+// kr_fused.h -- the fused MHD stage kernel: fluxes along x, y and z, edge
+// EMFs, flux-divergence update and constrained transport in one launch.
+//
+// One workgroup owns a TX x TY column tile of one block and marches through
+// the block along z.  Primitives W (written by k_c2p) are staged plane by
+// plane into a 3-plane LDS ring over the tile plus a 2-cell halo; HBM holds
+// no fluxes and no EMFs.  Per plane iteration L (leading plane):
+//   1. W(L) enters the ring (prefetched during the previous iteration);
+//   2. fluxes: in-plane x/y faces of plane L-1 and z faces at L-1 (one
+//      lane per column keeps its z pencil state in registers);
+//   3. EMFs: ee_z on plane L-1, ee_x / ee_y on z-face L-1;
+//   4. update: Heun slot of the cells of plane L-2 (x, y parts were
+//      accumulated one iteration earlier, z part now) and CT of the faces of
+//      plane L-2 and z-face L-1.
+// Arithmetic is the same per-face restatement as the staged kernels
+// (kr_physics.h), in the same order, so both paths agree bit for bit in the
+// FP64 form.  PLM is evaluated per cell: the reference's two face states of
+// a cell share one limited increment t, q_l(face i+1) = q_i + t and
+// q_r(face i) = q_i + (-t) exactly (hydro/reconstruct/plm.h:116-153).
+#pragma once
+
+#include "kr_staged.h"
+
+namespace kr {
+
+template <int TX, int TY>
+struct TileShape {
+    static constexpr int HX = TX + 4, HY = TY + 4, NCOL = HX * HY;    // halo columns
+    static constexpr int NFX = (TX + 1) * (TY + 2);                   // x faces, rows -1..TY
+    static constexpr int NFY = (TX + 2) * (TY + 1);                   // y faces, cols -1..TX
+    static constexpr int NF = NFX + NFY;
+    static constexpr int RX = TX + 2, NZG = (TX + 2) * (TY + 2);      // z-face ring grid
+    static constexpr int NZI = TX * TY + 2 * (TX + 2) + 2 * TY;       // ring items
+    static constexpr int NEZ = (TX + 1) * (TY + 1), NEX = TX * (TY + 1), NEY = (TX + 1) * TY;
+    __device__ static int col(int x, int y) { return (y + 2) * HX + (x + 2); }
+    __device__ static int zg(int x, int y) { return (y + 1) * RX + (x + 1); }
+    __device__ static int fxi(int x, int y) { return (y + 1) * (TX + 1) + x; }
+    __device__ static int fyi(int x, int y) { return NFX + y * (TX + 2) + (x + 1); }
+    __device__ static int ezi(int x, int y) { return y * (TX + 1) + x; }
+    __device__ static int exi(int x, int y) { return y * TX + x; }
+    __device__ static int eyi(int x, int y) { return y * (TX + 1) + x; }
+    // ring item r -> column (interior first, then rows -1 / TY, then cols -1 / TX)
+    __device__ static void ring_item(int r, int &x, int &y) {
+        if (r < TX * TY) {
+            x = r % TX;
+            y = r / TX;
+            return;
+        }
+        r -= TX * TY;
+        if (r < 2 * (TX + 2)) {
+            x = r % (TX + 2) - 1;
+            y = r < TX + 2 ? -1 : TY;
+            return;
+        }
+        r -= 2 * (TX + 2);
+        x = r < TY ? -1 : TX;
+        y = r % TY;
+    }
+};
+
+__device__ __forceinline__ int ring3(int p) { return ((p % 3) + 3) % 3; }
+__device__ __forceinline__ int ring2(int p) { return (p + 4) & 1; }
+
+// Per-cell PLM increment of one variable (see header): returns t with the
+// reference's expression order; `flat` is the reference's dq2 <= 0 branch.
+template <class FT>
+__device__ __forceinline__ FT plm_inc(FT qm, FT q0, FT qp, bool &flat) {
+    const FT dql = q0 - qm;
+    const FT dqc = qp - q0;
+    FT norm(1);
+    if constexpr (std::is_same_v<FT, float>) {
+        // FP32: the slope ratio is scale free; rescale by the larger
+        // difference so the squares stay in range (plm.h:128-137 in spirit)
+        const FT s = pmax(kfabs(dql), kfabs(dqc));
+        if (s > (FT)1e-24f) norm = 1 / s;
+    }
+    const FT dql2 = norm * dql * dql;
+    const FT dqc2 = norm * dqc * dqc;
+    const FT dq2 = norm * dql * dqc;
+    flat = dq2 <= 0;
+    const FT coef = flat ? (FT)0 : dq2 / (dql2 + dqc2 + dq2 * 2);
+    return coef * (dql + dqc);
+}
+
+// Edge EMF from the four surrounding cell E values and the two pairs of
+// face EMFs / upwind mass fluxes (mhd/integrator/integrator.h:244-317).
+// ec[s2][s1] naming of the reference: ec11 = E(i), ec10 = E(i - e1),
+// ec00 = E(i - e1 - e2), ec01 = E(i - e2).
+template <class FT>
+__device__ __forceinline__ FT edge_emf(FT ec11, FT ec10, FT ec00, FT ec01, FT efm1, FT efp1,
+                                       FT rp1, FT rm1, FT efm2, FT efp2, FT rp2, FT rm2) {
+    const FT w_e = (FT)0.25f;
+    FT ee = 0;
+    ee += w_e * (efm1 + efp1);
+    {
+        const FT de_p = (rp1 > 0 ? (ec01 - efm1) : (ec11 - efp1));
+        const FT de_m = (rm1 > 0 ? -(ec00 - efm1) : -(ec10 - efp1));
+        ee += w_e * (de_m - de_p);
+    }
+    ee += w_e * (efm2 + efp2);
+    {
+        const FT de_p = (rp2 > 0 ? (ec10 - efm2) : (ec11 - efp2));
+        const FT de_m = (rm2 > 0 ? -(ec00 - efm2) : -(ec01 - efp2));
+        ee += w_e * (de_m - de_p);
+    }
+    return ee;
+}
+
+template <class FT, int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
+    using S = TileShape<TX, TY>;
+    static_assert(S::NCOL <= NT, "one halo column per thread");
+    static_assert(S::NZI <= NT, "one z pencil per thread");
+    static_assert(TX * TY + TX + TY <= NT, "one owned cell / edge face per thread");
+    if (m.ts->skip) return;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.y;
+    const int ntx = m.n[0] / TX;
+    const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
+    const int nz = m.n[2];
+    const bool xlast = x0 + TX == m.n[0], ylast = y0 + TY == m.n[1];
+    const BlockGeo &g = m.geo[b];
+    const double dt = m.ts->dt_use * (step == 0 ? 1.0 : 0.5);
+    const FT gam = (FT)m.gamma;
+    const size_t nc = m.ncell;
+
+    __shared__ FT sW[3][8][S::NCOL];
+    __shared__ FT sFe[2][3][S::NF];     // rho, 2 EMF slots of x / y faces
+    __shared__ FT sFh[5][S::NF];        // hydro fluxes of the current plane
+    __shared__ FT sFz[3][S::NZG];       // rho, E_x, E_y of the current z face
+    __shared__ FT sEz[2][S::NEZ];
+    __shared__ FT sEx[2][S::NEX];
+    __shared__ FT sEy[2][S::NEY];
+
+    // ---- primitive source of halo column (cx, cy) at plane z
+    auto wsrc = [&](int cx, int cy, int z) -> const FT * {
+        int p[3] = {x0 + cx, y0 + cy, z};
+        int d[3];
+        bool out = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            d[a] = p[a] < 0 ? -1 : (p[a] >= m.n[a] ? 1 : 0);
+            out |= d[a] != 0;
+        }
+        int blk = b;
+        if (out) {
+            blk = g.nbr[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
+        }
+        return m.W + (size_t)blk * 8 * nc + cidx(m.n, p[0], p[1], p[2]);
+    };
+    auto bfn = [&](int c, int fx, int fy, int fz) -> FT {
+        const int p[3] = {x0 + fx, y0 + fy, fz};
+        return (FT)load_bf(m, b, step, c, p);
+    };
+
+    FT wpre[8];
+    const bool wloader = tid < S::NCOL;
+    const int wcx = tid % S::HX - 2, wcy = tid / S::HX - 2;
+    auto prefetch_w = [&](int z) {
+        if (!wloader) return;
+        const FT *s = wsrc(wcx, wcy, z);
+#pragma unroll
+        for (int v = 0; v < 8; ++v) wpre[v] = s[(size_t)v * nc];
+    };
+
+    // ---- per-thread persistent state
+    // z pencil owner (tid < NZI): column (zx, zy)
+    int zx = 0, zy = 0;
+    const bool zown = tid < S::NZI;
+    if (zown) S::ring_item(tid, zx, zy);
+    const bool zcorner = (zx < 0 || zx >= TX) && (zy < 0 || zy >= TY);
+    // owned cell: only pencil threads own one (the others keep zx = zy = 0)
+    const bool zint = zown && zx >= 0 && zx < TX && zy >= 0 && zy < TY;
+    FT qL[8];                 // left state of the next z face (mesh frame)
+    FT fzp[5];                // hydro z flux of the previous face (owned cells)
+    double part[5];           // x + y partial update of the previous plane
+#pragma unroll
+    for (int v = 0; v < 8; ++v) qL[v] = 0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        fzp[v] = 0;
+        part[v] = 0;
+    }
+    const int cx_ = zx, cy_ = zy;                 // owned cell (when zint)
+    // edge faces of the tile's high sides: threads TX*TY .. +TY-1 (x faces at
+    // x = TX) and +TY .. +TY+TX-1 (y faces at y = TY)
+    const int et = tid - TX * TY;
+    const bool xedge = xlast && et >= 0 && et < TY;
+    const bool yedge = ylast && et >= TY && et < TY + TX;
+
+    const double dt_dv = dt / g.vol;
+
+    prefetch_w(-2);
+    for (int L = -2; L <= nz + 1; ++L) {
+        // ------------------------------------------------ 1. ring store
+        if (wloader) {
+            const int s = ring3(L);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) sW[s][v][tid] = wpre[v];
+        }
+        __syncthreads();
+        if (L + 1 <= nz + 1) prefetch_w(L + 1);
+
+        // prefetch the conserved state of the plane being started (L-1) and
+        // the face fields of the CT of this iteration
+        const int P = L - 2;          // plane finished in this iteration
+        const int Pn = L - 1;         // plane started (x/y partial)
+        double us[5] = {0, 0, 0, 0, 0}, ut[5] = {0, 0, 0, 0, 0};
+        if (zint && Pn >= 0 && Pn < nz) {
+            const size_t c = cidx(m.n, x0 + cx_, y0 + cy_, Pn);
+            const double *Us = m.U + ((size_t)b * 2 + step) * 5 * nc + c;
+            const double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + c;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) us[v] = Us[(size_t)v * nc];
+            if (step == 1) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) ut[v] = Ut[(size_t)v * nc];
+            }
+        }
+
+        // ------------------------------------------------ 2. fluxes
+        if (L >= 0) {
+            const int p = L - 1;                 // plane of the in-plane faces / z face index
+            const int s0 = ring3(p), sm = ring3(p - 1), sp = ring3(p + 1);
+            const int fe = ring2(p);
+            // z pencil: increment of cell p, face states, flux at face p
+            if (zown && !zcorner) {
+                const int c = S::col(zx, zy);
+                FT qR[8];
+                FT qLn[8];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    if (v == BCC + 2) {
+                        qR[v] = 0;
+                        qLn[v] = 0;
+                        continue;
+                    }
+                    bool flat;
+                    const FT q0 = sW[s0][v][c];
+                    const FT t = plm_inc<FT>(sW[sm][v][c], q0, sW[sp][v][c], flat);
+                    qR[v] = q0 + (flat ? (FT)0 : -t);
+                    qLn[v] = q0 + (flat ? (FT)0 : t);
+                }
+                if (p >= 0 && p <= nz) {
+                    FT wl[8], wr[8], flx[8];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        wl[v] = qL[v];
+                        wr[v] = qR[v];
+                    }
+                    const FT bn = bfn(2, zx, zy, p);
+                    wl[BCC + 2] = bn;
+                    wr[BCC + 2] = bn;
+                    rot<2>(wl + MOM);
+                    rot<2>(wr + MOM);
+                    rot<2>(wl + BCC);
+                    rot<2>(wr + BCC);
+                    hlld<FT>(flx, wl, wr, gam);
+                    rot<1>(flx + MOM);
+                    // EMF rewrite for axis 2: E_x = flx[BCC+2], E_y = -flx[BCC+1]
+                    const int gz = S::zg(zx, zy);
+                    sFz[0][gz] = flx[RHO];
+                    sFz[1][gz] = flx[BCC + 2];
+                    sFz[2][gz] = -flx[BCC + 1];
+                    if (zint) {
+                        // z part of the cell finished now (plane p-1 = P)
+                        if (P >= 0) {
+                            const FT fa = (FT)(g.sfc[2] * dt_dv);
+                            const size_t cc = cidx(m.n, x0 + cx_, y0 + cy_, P);
+                            double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + cc;
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) {
+                                double res(0);
+                                if constexpr (std::is_same_v<FT, double>) {
+                                    if (fa > 0) res += fa * fzp[v];
+                                    if (fa > 0) res -= fa * flx[v];
+                                } else {
+                                    if (fa > 0) res += (double)(fa * fzp[v]);
+                                    if (fa > 0) res -= (double)(fa * flx[v]);
+                                }
+                                Ut[(size_t)v * nc] = part[v] + res;
+                            }
+                        }
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) fzp[v] = flx[v];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) qL[v] = qLn[v];
+            }
+            // in-plane faces of plane p: items beyond the z pencils
+            for (int it = tid - S::NZI + (tid < S::NZI ? NT : 0); it < S::NF; it += NT) {
+                if (it < 0) continue;
+                const bool xf = it < S::NFX;
+                int fx, fy;
+                if (xf) {
+                    fx = it % (TX + 1);
+                    fy = it / (TX + 1) - 1;
+                } else {
+                    const int k = it - S::NFX;
+                    fx = k % (TX + 2) - 1;
+                    fy = k / (TX + 2);
+                }
+                const bool ghost_plane = p < 0 || p >= nz;
+                if (ghost_plane && (xf ? (fy < 0 || fy >= TY) : (fx < 0 || fx >= TX))) continue;
+                const int a = xf ? 0 : 1;
+                const int stride = xf ? 1 : S::HX;
+                const int cR = S::col(fx, fy);          // cell on the right of the face
+                FT wl[8], wr[8], flx[8];
+                // rotated frame: slot k holds mesh variable perm(k)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    int v = k;
+                    if (k >= MOM && k < MOM + 3) v = MOM + (a + k - MOM) % 3;
+                    if (k >= BCC) v = BCC + (a + k - BCC) % 3;
+                    if (k == BCC) continue;            // normal field: face value
+                    const FT *q = &sW[s0][v][0];
+                    bool fl, fr;
+                    const FT qm2 = q[cR - 2 * stride], qm1 = q[cR - stride];
+                    const FT q00 = q[cR], qp1 = q[cR + stride];
+                    const FT tl = plm_inc<FT>(qm2, qm1, q00, fl);
+                    const FT tr = plm_inc<FT>(qm1, q00, qp1, fr);
+                    wl[k] = qm1 + (fl ? (FT)0 : tl);
+                    wr[k] = q00 + (fr ? (FT)0 : -tr);
+                }
+                const FT bn = bfn(a, fx, fy, p);
+                wl[BCC] = bn;
+                wr[BCC] = bn;
+                hlld<FT>(flx, wl, wr, gam);
+                // back to the mesh frame
+                sFh[0][it] = flx[RHO];
+                sFh[1][it] = flx[ENE];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sFh[2 + (a + k) % 3][it] = flx[MOM + k];
+                sFe[fe][0][it] = flx[RHO];
+                if (xf) {                        // x faces: E_y, E_z
+                    sFe[fe][1][it] = flx[BCC + 2];
+                    sFe[fe][2][it] = -flx[BCC + 1];
+                } else {                         // y faces: E_x, E_z
+                    sFe[fe][1][it] = -flx[BCC + 1];
+                    sFe[fe][2][it] = flx[BCC + 2];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ------------------------------------------------ 3. EMFs
+        if (L >= 1) {
+            const int p = L - 1;
+            const int s0 = ring3(p), sm = ring3(p - 1);
+            const int fe = ring2(p), fm = ring2(p - 1);
+            auto ec = [&](int s, int x, int y, int c) -> FT {
+                const int cc = S::col(x, y);
+                const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                return sW[s][BCC + c1][cc] * sW[s][MOM + c2][cc] -
+                       sW[s][BCC + c2][cc] * sW[s][MOM + c1][cc];
+            };
+            for (int it = tid; it < S::NEZ + S::NEX + S::NEY; it += NT) {
+                if (it < S::NEZ) {
+                    if (p >= nz) continue;
+                    const int x = it % (TX + 1), y = it / (TX + 1);
+                    // ee_z on plane p: a1 = x, a2 = y
+                    const FT e = edge_emf<FT>(
+                        ec(s0, x, y, 2), ec(s0, x - 1, y, 2), ec(s0, x - 1, y - 1, 2), ec(s0, x, y - 1, 2),
+                        sFe[fe][2][S::fxi(x, y - 1)], sFe[fe][2][S::fxi(x, y)],
+                        sFe[fe][0][S::fyi(x, y)], sFe[fe][0][S::fyi(x - 1, y)],
+                        sFe[fe][2][S::fyi(x - 1, y)], sFe[fe][2][S::fyi(x, y)],
+                        sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
+                    sEz[fe][S::ezi(x, y)] = e;
+                } else if (it < S::NEZ + S::NEX) {
+                    const int k = it - S::NEZ;
+                    const int x = k % TX, y = k / TX;
+                    // ee_x on z face p: a1 = y, a2 = z
+                    const FT e = edge_emf<FT>(
+                        ec(s0, x, y, 0), ec(s0, x, y - 1, 0), ec(sm, x, y - 1, 0), ec(sm, x, y, 0),
+                        sFe[fm][1][S::fyi(x, y)], sFe[fe][1][S::fyi(x, y)],
+                        sFz[0][S::zg(x, y)], sFz[0][S::zg(x, y - 1)],
+                        sFz[1][S::zg(x, y - 1)], sFz[1][S::zg(x, y)],
+                        sFe[fe][0][S::fyi(x, y)], sFe[fm][0][S::fyi(x, y)]);
+                    sEx[fe][S::exi(x, y)] = e;
+                } else {
+                    const int k = it - S::NEZ - S::NEX;
+                    const int x = k % (TX + 1), y = k / (TX + 1);
+                    // ee_y on z face p: a1 = z, a2 = x
+                    const FT e = edge_emf<FT>(
+                        ec(s0, x, y, 1), ec(sm, x, y, 1), ec(sm, x - 1, y, 1), ec(s0, x - 1, y, 1),
+                        sFz[2][S::zg(x - 1, y)], sFz[2][S::zg(x, y)],
+                        sFe[fe][0][S::fxi(x, y)], sFe[fm][0][S::fxi(x, y)],
+                        sFe[fm][1][S::fxi(x, y)], sFe[fe][1][S::fxi(x, y)],
+                        sFz[0][S::zg(x, y)], sFz[0][S::zg(x - 1, y)]);
+                    sEy[fe][S::eyi(x, y)] = e;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ------------------------------------------------ 4. update
+        if (L >= 1) {
+            const int q = L - 1;                     // z face updated now
+            const int fq = ring2(q), fpp = ring2(P);
+            auto set_face = [&](int c, int fx, int fy, int fz, double db) {
+                const size_t f = fidx(m.n, x0 + fx, y0 + fy, fz);
+                double *Bs = m.BF + (((size_t)b * 2 + step) * 3 + c) * m.nface + f;
+                double *Bt = m.BF + (((size_t)b * 2 + (1 - step)) * 3 + c) * m.nface + f;
+                const double bs = *Bs;
+                *Bt = db + (step == 0 ? bs : (bs + *Bt) / 2);
+            };
+            // CT circulation of one face (mhd/integrator/integrator.h:429-457)
+            auto ct = [&](double e1, double d1, double e2, double d2, double e3, double e4,
+                          double sfc) {
+                double db = e1 * d1;
+                db -= e2 * d2;
+                db += e3 * d2;
+                db -= e4 * d1;
+                db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
+                return db;
+            };
+            const double dx = g.dx[0], dy = g.dx[1], dz = g.dx[2];
+            if (zint) {
+                const int x = cx_, y = cy_;
+                // x + y partial of plane q (cells), finished next iteration
+                if (q < nz) {
+                    double uu[5];
+                    if (step == 0) {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) uu[v] = us[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) uu[v] = (us[v] + ut[v]) * 0.5f;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int i0 = a == 0 ? S::fxi(x, y) : S::fyi(x, y);
+                        const int i1 = a == 0 ? S::fxi(x + 1, y) : S::fyi(x, y + 1);
+                        if constexpr (std::is_same_v<FT, double>) {
+                            const double fa = g.sfc[a] * dt_dv;
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) {
+                                double res(0);
+                                if (fa > 0) res += fa * sFh[v][i0];
+                                if (fa > 0) res -= fa * sFh[v][i1];
+                                uu[v] += res;
+                            }
+                        } else {
+                            const float fa = (float)(g.sfc[a] * dt_dv);
+#pragma unroll
+                            for (int v = 0; v < 5; ++v) {
+                                double res(0);
+                                if (fa > 0) res += (double)(fa * sFh[v][i0]);
+                                if (fa > 0) res -= (double)(fa * sFh[v][i1]);
+                                uu[v] += res;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) part[v] = uu[v];
+                }
+                // z face q
+                set_face(2, x, y, q,
+                         ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
+                            sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], g.sfc[2]));
+                if (P >= 0) {
+                    // x face (x, y, P): a1 = y, a2 = z
+                    set_face(0, x, y, P,
+                             ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
+                                sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], g.sfc[0]));
+                    // y face (x, y, P): a1 = z, a2 = x
+                    set_face(1, x, y, P,
+                             ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
+                                sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], g.sfc[1]));
+                }
+            }
+            if (P >= 0 && xedge) {
+                const int x = TX, y = et;
+                set_face(0, x, y, P,
+                         ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
+                            sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], g.sfc[0]));
+            }
+            if (P >= 0 && yedge) {
+                const int x = et - TY, y = TY;
+                set_face(1, x, y, P,
+                         ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
+                            sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], g.sfc[1]));
+            }
+        }
+    }
+}
+
+}  // namespace kr

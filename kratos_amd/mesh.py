@@ -188,7 +188,8 @@ class Physics:
 class Context:
     """One device context of the C-ABI over a uniform block mesh."""
 
-    def __init__(self, spec: MeshSpec, phys: Physics, precision="fp64"):
+    def __init__(self, spec: MeshSpec, phys: Physics, precision="fp64", path="auto",
+                 tile=(0, 0)):
         lib = _capi.load()
         self.lib, self.spec, self.phys = lib, spec, phys
         self.mhd = phys.mhd
@@ -204,6 +205,8 @@ class Context:
         cfg.top_up = int(phys.top_up)
         for a in range(3):
             cfg.n_cell_block[a] = spec.n_cell_block[a]
+        cfg.kernel_path = _capi.PATHS[path]
+        cfg.tile[0], cfg.tile[1] = int(tile[0]), int(tile[1])
         h = C.c_void_p()
         rc = lib.kr_create(C.byref(cfg), C.byref(h))
         _capi.check(rc, None, "kr_create")

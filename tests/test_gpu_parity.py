@@ -18,10 +18,10 @@ def rel_l1(a, b):
     return d / s if s > 0 else d
 
 
-def run_both(spec, phys, u, b, n_cycles, precision="fp64"):
+def run_both(spec, phys, u, b, n_cycles, precision="fp64", path="auto"):
     print(f"\n[case] N={spec.n_cell_global} blk={spec.n_cell_block} mhd={phys.mhd} "
-          f"prec={precision} cycles={n_cycles}", flush=True)
-    ctx = kratos_amd.Context(spec, phys, precision)
+          f"prec={precision} path={path} cycles={n_cycles}", flush=True)
+    ctx = kratos_amd.Context(spec, phys, precision, path=path)
     ctx.upload(u, b)
     done = ctx.advance(n_cycles)
     gu, gb = ctx.download()
@@ -48,13 +48,14 @@ def test_sod_fp64_every_cell(gpu_lib):
     np.testing.assert_array_equal(gu, ou)
 
 
-@pytest.mark.parametrize("blk", [(16, 16, 16), (8, 16, 32)])
-def test_mhd_turbulence_fp64(gpu_lib, blk):
-    N = (32, 32, 32)
+@pytest.mark.parametrize("path", ["staged", "fused"])
+@pytest.mark.parametrize("N,blk", [((32, 32, 32), (16, 16, 16)), ((32, 32, 32), (8, 16, 32)),
+                                   ((64, 64, 64), (32, 32, 32))])
+def test_mhd_turbulence_fp64(gpu_lib, N, blk, path):
     spec = kratos_amd.MeshSpec(N, blk)
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     u, b, _ = problems.turbulence(N, seed=7)
-    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, path=path)
     assert gt[0] == ot[0]
     for v in range(5):
         assert rel_l1(gu[v], ou[v]) <= 1e-12, v
@@ -62,15 +63,16 @@ def test_mhd_turbulence_fp64(gpu_lib, blk):
         assert rel_l1(gb[a], ob[a]) <= 1e-12, a
 
 
+@pytest.mark.parametrize("path", ["staged", "fused"])
 @pytest.mark.parametrize("beta", [1.0, 0.01])
-def test_mhd_turbulence_mixed(gpu_lib, beta):
+def test_mhd_turbulence_mixed(gpu_lib, beta, path):
     """Mixed form (FP32 reconstruction/Riemann/EMF, FP64 state, CT, c2p, dt)
     against the FP64 oracle: relative L1 of rho, E and B within 1e-5."""
     N = (32, 32, 32)
     spec = kratos_amd.MeshSpec(N, (16, 16, 16))
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     u, b, _ = problems.turbulence(N, seed=11, beta=beta)
-    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, "mixed")
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, "mixed", path=path)
     assert abs(gt[0] - ot[0]) <= 1e-5 * ot[0]
     assert rel_l1(gu[0], ou[0]) <= 1e-5
     assert rel_l1(gu[1], ou[1]) <= 1e-5
@@ -78,13 +80,14 @@ def test_mhd_turbulence_mixed(gpu_lib, beta):
         assert rel_l1(gb[a], ob[a]) <= 1e-5, a
 
 
-def test_mhd_single_block_self_periodic(gpu_lib):
+@pytest.mark.parametrize("path", ["staged", "fused"])
+def test_mhd_single_block_self_periodic(gpu_lib, path):
     """One 16^3 block that is its own 26 neighbours (the config-5 box shape)."""
     N = (16, 16, 16)
     spec = kratos_amd.MeshSpec(N, N)
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     u, b, _ = problems.turbulence(N, seed=5, beta=0.3)
-    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 12)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 12, path=path)
     np.testing.assert_array_equal(gu, ou)
     for a in range(3):
         np.testing.assert_array_equal(gb[a], ob[a])

@@ -1,0 +1,200 @@
+"""Pins the oracle's per-face functors against what the mathematics fixes.
+
+The reference ships no tests, golden vectors or fixtures for this path (its
+verification programs live in a private repository that is not present), so
+each functor is pinned by closed forms and identities chosen so that a dropped
+term, a wrong sign or a swapped operand fails at least one of them:
+
+* PLM (hydro/reconstruct/plm.h:116-153): linear data reconstructed exactly,
+  extrema flattened, the van-Leer harmonic slope against a direct formula.
+* HLLC (hydro/riemann/hllc.h:76-193): consistency F(U, U) = f(U), supersonic
+  upwinding returns the physical flux of the upwind state, a stationary
+  contact carries only the pressure, mirror symmetry.
+* HLLD (mhd/riemann/hlld.h:81-298): consistency, supersonic upwinding, an
+  isolated stationary contact, mirror symmetry, and the face-EMF rewrite
+  (hlld.h:42-46) against E = -v x B for a uniform state.
+* c2p (hydro/eos/adiabatic.h:159-206): primitive -> conserved -> primitive
+  round trip, density / pressure floors and the healed energy.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+G = 5.0 / 3.0
+RNG = np.random.default_rng(1234)
+
+
+def hydro_flux(w, gamma):
+    rho, p, u, v, ww = w
+    e = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + ww * ww)
+    return np.array([rho * u, (e + p) * u, rho * u * u + p, rho * u * v, rho * u * ww])
+
+
+def mhd_flux(w, gamma):
+    rho, p, u, v, ww, bx, by, bz = w
+    b2 = bx * bx + by * by + bz * bz
+    pt = p + 0.5 * b2
+    e = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + ww * ww) + 0.5 * b2
+    vb = u * bx + v * by + ww * bz
+    return np.array([rho * u, (e + pt) * u - bx * vb, rho * u * u + pt - bx * bx,
+                     rho * u * v - bx * by, rho * u * ww - bx * bz, 0.0,
+                     by * u - bx * v, bz * u - bx * ww])
+
+
+def rand_hydro():
+    return np.array([RNG.uniform(0.1, 3), RNG.uniform(0.1, 3), *RNG.uniform(-1, 1, 3)])
+
+
+def rand_mhd():
+    return np.array([RNG.uniform(0.1, 3), RNG.uniform(0.1, 3), *RNG.uniform(-1, 1, 3),
+                     *RNG.uniform(-2, 2, 3)])
+
+
+# ------------------------------------------------------------------ PLM
+def test_plm_linear_exact():
+    for a, b in [(0.0, 1.0), (2.0, -0.5), (1.0, 0.25)]:
+        q = np.array([a - b, a, a + b, a + 2 * b])       # cells i-2 .. i+1
+        ql, qr = oracle.plm(q)
+        assert ql == a + 0.5 * b                          # right edge of cell i-1
+        assert qr == a + b - 0.5 * b                      # left edge of cell i
+
+
+def test_plm_extremum_flat():
+    ql, qr = oracle.plm(np.array([0.0, 1.0, 0.5, 2.0]))   # cell i-1 is a maximum
+    assert ql == 1.0
+    ql, qr = oracle.plm(np.array([3.0, 1.0, 0.5, 2.0]))   # cell i is a minimum
+    assert qr == 0.5
+
+
+def test_plm_harmonic_slope():
+    for _ in range(200):
+        q = RNG.uniform(-1, 1, 4)
+        ql, qr = oracle.plm(q)
+        dl, dc, dr = q[1] - q[0], q[2] - q[1], q[3] - q[2]
+        sl = dl * dc / (dl + dc) if dl * dc > 0 else 0.0
+        sr = dc * dr / (dc + dr) if dc * dr > 0 else 0.0
+        assert ql == pytest.approx(q[1] + sl, rel=1e-13, abs=1e-15)
+        assert qr == pytest.approx(q[2] - sr, rel=1e-13, abs=1e-15)
+        # TVD: face states stay inside the neighbouring cell values
+        assert min(q[0], q[1], q[2]) - 1e-15 <= ql <= max(q[0], q[1], q[2]) + 1e-15
+
+
+# ------------------------------------------------------------------ HLLC
+def test_hllc_consistency():
+    for _ in range(100):
+        w = rand_hydro()
+        f = oracle.hllc(w, w, 1.4)
+        np.testing.assert_allclose(f, hydro_flux(w, 1.4), rtol=1e-12, atol=1e-13)
+
+
+def test_hllc_supersonic_upwind():
+    wl = np.array([1.0, 1.0, 5.0, 0.3, -0.2])
+    wr = np.array([0.5, 0.2, 4.0, -0.1, 0.4])
+    np.testing.assert_allclose(oracle.hllc(wl, wr, 1.4), hydro_flux(wl, 1.4), rtol=1e-14)
+    wl2, wr2 = wl.copy(), wr.copy()
+    wl2[2], wr2[2] = -4.0, -5.0
+    np.testing.assert_allclose(oracle.hllc(wl2, wr2, 1.4), hydro_flux(wr2, 1.4), rtol=1e-14)
+
+
+def test_hllc_stationary_contact():
+    wl = np.array([1.0, 0.7, 0.0, 0.0, 0.0])
+    wr = np.array([0.125, 0.7, 0.0, 0.0, 0.0])
+    f = oracle.hllc(wl, wr, 1.4)
+    np.testing.assert_allclose(f, [0.0, 0.0, 0.7, 0.0, 0.0], atol=1e-15, rtol=1e-15)
+
+
+def test_hllc_mirror_symmetry():
+    for _ in range(50):
+        wl, wr = rand_hydro(), rand_hydro()
+        f = oracle.hllc(wl, wr, 1.4)
+        ml, mr = wr.copy(), wl.copy()
+        ml[2] *= -1
+        mr[2] *= -1
+        g = oracle.hllc(ml, mr, 1.4)
+        # mass, energy and transverse momentum flip sign; normal momentum does not
+        np.testing.assert_allclose(g, [-f[0], -f[1], f[2], -f[3], -f[4]], rtol=1e-12, atol=1e-13)
+
+
+# ------------------------------------------------------------------ HLLD
+def test_hlld_consistency():
+    for _ in range(100):
+        w = rand_mhd()
+        f = oracle.hlld(w, w, G)
+        np.testing.assert_allclose(f, mhd_flux(w, G), rtol=1e-12, atol=1e-12)
+
+
+def test_hlld_supersonic_upwind():
+    wl = np.array([1.0, 0.1, 8.0, 0.3, -0.2, 0.5, 0.4, -0.3])
+    wr = np.array([0.5, 0.2, 7.0, -0.1, 0.4, 0.5, -0.2, 0.1])
+    np.testing.assert_allclose(oracle.hlld(wl, wr, G), mhd_flux(wl, G), rtol=1e-14, atol=1e-15)
+
+
+def test_hlld_stationary_contact():
+    wl = np.array([1.0, 0.6, 0.0, 0.0, 0.0, 0.8, 0.3, -0.2])
+    wr = np.array([0.2, 0.6, 0.0, 0.0, 0.0, 0.8, 0.3, -0.2])
+    f = oracle.hlld(wl, wr, G)
+    pt = 0.6 + 0.5 * (0.64 + 0.09 + 0.04)
+    exp = [0.0, 0.0, pt - 0.64, -0.8 * 0.3, 0.8 * 0.2, 0.0, 0.0, 0.0]
+    np.testing.assert_allclose(f, exp, atol=1e-14)
+
+
+def test_hlld_mirror_symmetry():
+    for _ in range(50):
+        wl, wr = rand_mhd(), rand_mhd()
+        wr[5] = wl[5]
+        f = oracle.hlld(wl, wr, G)
+        ml, mr = wr.copy(), wl.copy()
+        for m in (ml, mr):
+            m[2] *= -1          # normal velocity
+            m[5] *= -1          # normal field
+        g = oracle.hlld(ml, mr, G)
+        # mass, energy, transverse momentum and the transverse induction
+        # fluxes (B_t u - B_n v_t) are odd under x -> -x; normal momentum is even
+        exp = [-f[0], -f[1], f[2], -f[3], -f[4], 0.0, -f[6], -f[7]]
+        np.testing.assert_allclose(g, exp, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_face_emf_rewrite(axis):
+    """A uniform state's face flux carries E = -v x B in the two tangential
+    B slots and 0 in the normal slot, in the mesh frame (hlld.h:29-47)."""
+    w = np.array([1.3, 0.9, 0.4, -0.7, 0.2, 0.5, -1.1, 0.8])
+    f = oracle.face_flux(True, axis, w, w, G)
+    v, b = w[2:5], w[5:8]
+    e = -np.cross(v, b)
+    a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+    assert f[5 + axis] == 0.0
+    assert f[5 + a1] == pytest.approx(e[a1], rel=1e-13)
+    assert f[5 + a2] == pytest.approx(e[a2], rel=1e-13)
+    # momentum flux back in the mesh frame: rho v_axis v_j + p_tot delta - B_axis B_j
+    pt = w[1] + 0.5 * b @ b
+    for j in range(3):
+        exp = w[0] * v[axis] * v[j] + (pt if j == axis else 0.0) - b[axis] * b[j]
+        assert f[2 + j] == pytest.approx(exp, rel=1e-12)
+
+
+# ------------------------------------------------------------------ c2p
+def test_c2p_round_trip():
+    for _ in range(200):
+        rho, p = RNG.uniform(0.1, 3, 2)
+        v = RNG.uniform(-2, 2, 3)
+        u = np.array([rho, p / (G - 1) + 0.5 * rho * v @ v, *(rho * v)])
+        u2, w, cs2 = oracle.c2p_hydro(u, G)
+        np.testing.assert_array_equal(u2, u)
+        np.testing.assert_allclose(w, [rho, p, *v], rtol=1e-12)
+        assert cs2 == pytest.approx(G * p / rho, rel=1e-12)
+
+
+def test_c2p_floors_heal():
+    u = np.array([-1.0, 1.0, 0.0, 0.0, 0.0])
+    u2, w, _ = oracle.c2p_hydro(u, G, rho_floor=1e-6, tmp_floor=1e-3)
+    assert u2[0] == 1e-6 and w[0] == 1e-6
+    u = np.array([1.0, -5.0, 0.1, 0.0, 0.0])
+    u2, w, cs2 = oracle.c2p_hydro(u, G, rho_floor=1e-9, tmp_floor=1e-3)
+    assert w[1] == 1e-3 * 1.0
+    assert u2[1] == pytest.approx(0.5 * 0.01 + 1e-3 / (G - 1), rel=1e-15)
+    assert cs2 == pytest.approx(G * 1e-3, rel=1e-15)
+    u = np.array([np.nan, 1.0, 0.0, 0.0, 0.0])
+    u2, w, _ = oracle.c2p_hydro(u, G, rho_floor=1e-7)
+    assert u2[0] == 1e-7
