@@ -10,13 +10,26 @@ hipGraph, bracketed by device synchronisation.
 
 Both precision forms run: the plain FP64 form and the mixed form (FP32
 reconstruction / Riemann / EMF, FP64 conserved state, CT, energy and dt).
-``value`` is the mixed form's rate; the FP64 rate, the speed-up and the
-mixed-vs-FP64 L1 errors are reported beside it.
+``value`` is the mixed form's rate; the FP64 rate and the speed-up are
+reported beside it.
+
+accuracy: the same recipe at beta = 1 and beta = 0.01, both forms run to
+t = 0.1 (t_lim top-up); relative L1 of rho, E and B of the mixed form
+against the FP64 form (which equals the CPU oracle bit for bit on every
+case the parity tests run), and max |div B|.
+
+ensemble: 64 independent 64^3 periodic boxes (seeds 0..63, beta drawn from
+U[0.1, 10]) in one context, one time group per box (per-box dt), all boxes
+advanced by the same launches to t = 0.05; aggregate cell-updates/s of the
+mixed form and the per-box relative density L1 against the FP64 form.
 
 roofline: the dominant kernel (largest share of the cycle) with its
 algorithmic bytes per launch divided by its mean launch time (HIP events on
-the library's stream).  cpu_baseline: the CPU oracle (FP64 C++ restatement of
-the reference algorithm, "port") on a bounded sample of the same workload.
+the library's stream).  traffic: HBM bytes per launch of that kernel from
+the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
+for the same kernel and workload, when present.  cpu_baseline: the CPU
+oracle (FP64 C++ restatement of the reference algorithm, "port") on a
+bounded sample of the same recipe.
 """
 import argparse
 import json
@@ -30,14 +43,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STATE_DOUBLES = 5 + 3          # conserved rho, E, m + one face B per axis
 
-# Algorithmic HBM bytes per launch unit (cell or face) of each staged kernel,
-# per FP64/FP32 form: reads + writes of the arrays the kernel must touch once.
-#   c2p    : U (5 x 8 B) + 6 face-B reads (3 distinct x 8 B) ; W write (8 x fs)
-#   flux   : W read (8 x fs, once per cell) ; F write (8 x fs per face)
-#   emf    : F reads (3 axes x 3 slots x fs) + W (6 x fs) ; EE write (3 x fs)
-#   update : U src (+tgt at substep 1) + F reads + EE + BF r/w
-def alg_bytes(kind, fs, step_avg=True):
+
+def alg_bytes(kind, fs, fused):
+    """Algorithmic HBM bytes per launch unit (cell, or face for the staged
+    flux kernels) of one kernel kind, averaged over the two RK substeps.
+
+    fused stage kernel ("update" in the fused path): the state it must read
+    and write once -- substep 0 reads U, BF of slot 0 and writes U, BF of
+    slot 1 (2 x 8 doubles); substep 1 also reads the target slot (3 x 8).
+    Staged kernels: their own inputs and outputs read / written once.
+    """
+    if kind == "update" and fused:
+        return (2 * STATE_DOUBLES + 3 * STATE_DOUBLES) / 2 * 8
     if kind == "c2p":
         return (5 + 3) * 8 + 8 * fs
     if kind.startswith("flux"):
@@ -45,19 +64,23 @@ def alg_bytes(kind, fs, step_avg=True):
     if kind == "emf":
         return 9 * fs + 6 * fs + 3 * fs
     if kind == "update":
-        u = (5 + 5) * 8 if not step_avg else 7.5 * 8 + 5 * 8
+        u = 7.5 * 8 + 5 * 8
         return u + 3 * 5 * fs + 3 * fs + (3 + 3) * 8
     return 0.0
 
 
-def make_problem(n, seed=0, beta=1.0):
+def make_problem(n, seed=0, beta=1.0, beta_range=None):
     from kratos_amd import problems
-    return problems.turbulence((n, n, n), seed=seed, beta=beta)
+    return problems.turbulence((n, n, n), seed=seed, beta=beta, beta_range=beta_range)
 
 
 def rel_l1(a, b):
     s = np.abs(b).sum()
     return float(np.abs(a - b).sum() / s) if s > 0 else float(np.abs(a - b).sum())
+
+
+def sync_all(ctx):
+    ctx.lib.kr_sync(ctx.h)
 
 
 def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="auto"):
@@ -66,21 +89,74 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
     ctx.upload(u, b)
     if warmup:
         ctx.advance(warmup)
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
-    ctx.lib.kr_sync(ctx.h)
+    sync_all(ctx)
     t0 = time.perf_counter()
     done = ctx.advance(steps)
-    ctx.lib.kr_sync(ctx.h)
+    sync_all(ctx)
     wall = time.perf_counter() - t0
     st = ctx.stats()
     prof = ctx.profile(profile_cycles) if profile_cycles else None
     return ctx, {"wall": wall, "cycles": done, "gpu_ms": st["gpu_ms"], "prof": prof,
                  "n_cells": st["n_cells"]}
+
+
+def accuracy_leg(n, nb, beta, t_lim, path):
+    """Both forms to t_lim from one IC; mixed-vs-FP64 L1 and div B."""
+    import kratos_amd
+    spec = kratos_amd.MeshSpec((n, n, n), (nb, nb, nb))
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3, t_lim=t_lim, top_up=True)
+    u, b, _ = make_problem(n, beta=beta)
+    out = {}
+    for form in ("fp64", "mixed"):
+        ctx = kratos_amd.Context(spec, phys, form, path=path)
+        ctx.upload(u, b)
+        done = ctx.advance(1000000)
+        out[form] = (ctx.download(), ctx.divb_max(), ctx.time()[0], done)
+        ctx.close()
+    (um, bm), divb_m, tm, cm = out["mixed"]
+    (ud, bd), divb_d, td, cd = out["fp64"]
+    cat = lambda bs: np.concatenate([x.ravel() for x in bs])
+    return {"beta": beta, "t": tm, "cycles_mixed": cm, "cycles_fp64": cd,
+            "l1_rho": rel_l1(um[0], ud[0]), "l1_E": rel_l1(um[1], ud[1]),
+            "l1_B": rel_l1(cat(bm), cat(bd)),
+            "divb_max_mixed": divb_m, "divb_max_fp64": divb_d}
+
+
+def ensemble_leg(n_box, n, t_lim, path):
+    """n_box independent n^3 boxes, one time group each, both forms."""
+    import kratos_amd
+    spec = kratos_amd.MeshSpec((n, n, n), (n, n, n))
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3, t_lim=t_lim, top_up=True)
+    ics = [make_problem(n, seed=s, beta_range=(0.1, 10.0))[:2] for s in range(n_box)]
+    res = {}
+    for form in ("fp64", "mixed"):
+        ctx = kratos_amd.Context(spec, phys, form, path=path, members=n_box)
+        for m, (u, b) in enumerate(ics):
+            ctx.upload(u, b, member=m)
+        ctx.advance(1)                      # warm-up cycle (builds the graph)
+        sync_all(ctx)
+        c0 = [ctx.time(member=m)[2] for m in range(n_box)]
+        t0 = time.perf_counter()
+        ctx.advance(1000000)
+        sync_all(ctx)
+        wall = time.perf_counter() - t0
+        c1 = [ctx.time(member=m)[2] for m in range(n_box)]
+        upd = sum((c1[m] - c0[m]) * n ** 3 for m in range(n_box))
+        states = [ctx.download(member=m) for m in range(n_box)]
+        res[form] = {"rate": upd / wall, "wall": wall, "cycles": c1,
+                     "t": [ctx.time(member=m)[0] for m in range(n_box)],
+                     "divb": ctx.divb_max(), "states": states}
+        ctx.close()
+    l1 = [rel_l1(res["mixed"]["states"][m][0][0], res["fp64"]["states"][m][0][0])
+          for m in range(n_box)]
+    return {"boxes": n_box, "box_cells": n ** 3, "t": t_lim,
+            "cell_updates_per_s_mixed": res["mixed"]["rate"],
+            "cell_updates_per_s_fp64": res["fp64"]["rate"],
+            "wall_s_mixed": res["mixed"]["wall"],
+            "cycles_min_max": [min(res["mixed"]["cycles"]), max(res["mixed"]["cycles"])],
+            "all_boxes_at_t": all(abs(t - t_lim) == 0 for t in res["mixed"]["t"]),
+            "l1_rho_per_box_max": max(l1), "l1_rho_per_box_median": float(np.median(l1)),
+            "divb_max_mixed": res["mixed"]["divb"]}
 
 
 def cpu_baseline(seconds_target=15.0, threads=None):
@@ -106,6 +182,19 @@ def cpu_baseline(seconds_target=15.0, threads=None):
                       f"{wall:.1f} s, FP64 C++ oracle on {threads} threads"}
 
 
+def committed_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rows = json.load(f)
+    for r in rows:
+        if r.get("kernel") == kernel and r.get("workload") == workload:
+            return r.get("bytes_per_launch")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +207,12 @@ def main():
     ap.add_argument("--profile-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--t-acc", type=float, default=0.1)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-boxes", type=int, default=64)
+    ap.add_argument("--ens-n", type=int, default=64)
+    ap.add_argument("--t-ens", type=float, default=0.05)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -134,49 +229,62 @@ def main():
     print(f"[bench] IC {n}^3 generated in {t_ic:.1f} s", file=sys.stderr, flush=True)
 
     forms = [f for f in args.forms.split(",") if f]
-    res, states = {}, {}
+    res = {}
+    fused = None
     for form in forms:
         ctx, r = run_form(spec, phys, u, b, form, args.steps, args.warmup, args.profile_cycles,
                           args.path)
         r["rate"] = r["n_cells"] * r["cycles"] / r["wall"]
-        states[form] = ctx.download()
         r["divb"] = ctx.divb_max()
         r["t"] = ctx.time()[0]
+        if fused is None:
+            # the fused stage kernel reports under "update" with no flux launches
+            k = r["prof"]["kernel_calls"] if r["prof"] else {}
+            fused = k.get("update", 0) > 0 and k.get("flux_x", 0) == 0
         ctx.close()
         res[form] = r
-        print(f"[bench] {form}: {r['rate']:.4e} cell-updates/s, {r['wall'] / max(r['cycles'], 1) * 1e3:.3f}"
-              f" ms/cycle, divB {r['divb']:.2e}", file=sys.stderr, flush=True)
+        print(f"[bench] {form}: {r['rate']:.4e} cell-updates/s, "
+              f"{r['wall'] / max(r['cycles'], 1) * 1e3:.3f} ms/cycle, divB {r['divb']:.2e}",
+              file=sys.stderr, flush=True)
+    del u, b
 
     head = "mixed" if "mixed" in res else forms[0]
     r = res[head]
     fs = 4 if head == "mixed" else 8
     kms = r["prof"]["kernel_ms"] if r["prof"] else {}
     calls = r["prof"]["kernel_calls"] if r["prof"] else {}
-    # dominant kernel: largest total time per cycle
     dom, dom_t = None, -1.0
     for k, ms in kms.items():
         tot = ms * calls.get(k, 0)
         if tot > dom_t:
             dom, dom_t = k, tot
+    workload = f"mhd_turbulence_{n}^3_{nb}^3_{head}"
     roof = None
     if dom:
         nc = r["n_cells"]
         units = nc
         if dom.startswith("flux"):
             units = nc * (nb + 1) * (nb + 2) ** 2 / nb ** 3   # faces incl. transverse ghost lines
-        bytes_launch = alg_bytes(dom, fs) * units
+        bytes_launch = alg_bytes(dom, fs, fused) * units
         achieved = bytes_launch / (kms[dom] * 1e-3) / 1e9
+        kname = ("k_sweep (fused stage)" if fused and dom == "update" else f"k_{dom}")
+        traffic = committed_traffic(kname, workload)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": dom, "mean_launch_ms": round(kms[dom], 4),
-                "alg_bytes_per_launch": bytes_launch}
-    acc = {}
-    if "mixed" in states and "fp64" in states:
-        (um, bm), (ud, bd) = states["mixed"], states["fp64"]
-        acc = {"l1_rho": rel_l1(um[0], ud[0]), "l1_E": rel_l1(um[1], ud[1]),
-               "l1_B": rel_l1(np.concatenate([x.ravel() for x in bm]),
-                              np.concatenate([x.ravel() for x in bd])),
-               "t": res["mixed"]["t"]}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": kname, "mean_launch_ms": round(kms[dom], 4),
+                "alg_bytes_per_launch": bytes_launch,
+                "cycle_frac": round(STATE_DOUBLES * 5 * 8 * r["rate"] / 1e9 / HBM_PEAK_GBS, 4)}
+
+    acc = []
+    if not args.no_accuracy:
+        for beta in (1.0, 0.01):
+            a = accuracy_leg(n, nb, beta, args.t_acc, args.path)
+            acc.append(a)
+            print(f"[bench] accuracy beta={beta}: {a}", file=sys.stderr, flush=True)
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg(args.ens_boxes, args.ens_n, args.t_ens, args.path)
+        print(f"[bench] ensemble: {ens}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     line = {
         "metric": "MHD cell-updates/s (mixed vs FP64 at stated L1 error), % of HBM roofline",
@@ -193,14 +301,16 @@ def main():
         "data": "synthetic (seeded turbulence IC)",
         "config": {"workload": f"3-D MHD turbulence {n}^3, {spec.n_blocks} blocks of {nb}^3, "
                                "PLM+HLLD+CT+RK2, gamma 5/3, beta 1, Mach 0.5",
-                   "form": head, "cells": r["n_cells"]},
-        "forms": {f: {"cell_updates_per_s": v["rate"], "ms_per_cycle": v["wall"] / max(v["cycles"], 1) * 1e3,
+                   "form": head, "cells": r["n_cells"], "kernel_path": "fused" if fused else "staged"},
+        "forms": {f: {"cell_updates_per_s": v["rate"],
+                      "ms_per_cycle": v["wall"] / max(v["cycles"], 1) * 1e3,
                       "divb_max": v["divb"],
                       "kernel_ms": v["prof"]["kernel_ms"] if v["prof"] else None}
                   for f, v in res.items()},
         "speedup_mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]
                                     if "mixed" in res and "fp64" in res else None),
         "accuracy_mixed_vs_fp64": acc,
+        "ensemble": ens,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

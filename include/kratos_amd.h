@@ -54,6 +54,10 @@ enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };
 typedef struct kr_block_desc {
     int    id;             /* caller's block id                              */
     int    level;          /* refinement level (same-level meshes only)      */
+    int    group;          /* time group: one mesh with its own t and dt.
+                              0 for a single mesh; an ensemble of
+                              independent boxes uses 0 .. n_box-1, and blocks
+                              of different groups never neighbour.         */
     int    reg[3];         /* logical block index                            */
     int    n_ceff[3];      /* cells per axis (== kr_config.n_cell_block)     */
     double xf0[3], dx0[3]; /* geometry, as meshgen computes it               */
@@ -119,8 +123,16 @@ int kr_advance(kr_ctx *ctx, int n_cycles, int *n_done);
 
 /* Time and dt: cycle::dt() / t (mesh/cycle/cycle.cpp:117-122).           */
 int kr_get_time(kr_ctx *ctx, double *t, double *dt_last, double *dt_seed);
-/* Set t and the device dt seed (restart / cycle::init's dt_init).        */
+/* Set t and the device dt seed of every time group (restart /
+ * cycle::init's dt_init).                                                 */
 int kr_set_time(kr_ctx *ctx, double t, double dt_seed);
+
+/* t, the last dt taken and the cycles run by one time group: the per-mesh
+ * cycle::t / dt() (mesh/cycle/cycle.cpp:117-122) of one ensemble member,
+ * each member having its own global-dt reduction (cycle.cpp:91-115).
+ * kr_get_time reports group 0; kr_step / kr_advance count a cycle when
+ * any group ran it, and kr_advance stops once every group reached t_lim.  */
+int kr_get_group_time(kr_ctx *ctx, int group, double *t, double *dt_last, int *cycles);
 
 /* Max |div B| over all cells, slot 0 (the CT identity probe of
  * mhd/integrator/ct_id_dbg.h:10-40, as a diagnostic).                    */

@@ -40,6 +40,7 @@ class KrBlockDesc(C.Structure):
     _fields_ = [
         ("id", C.c_int),
         ("level", C.c_int),
+        ("group", C.c_int),
         ("reg", C.c_int * 3),
         ("n_ceff", C.c_int * 3),
         ("xf0", C.c_double * 3),
@@ -70,6 +71,8 @@ EXPORTS = {
     "kr_get_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]),
     "kr_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "kr_get_group_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kr_divb_max": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "kr_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kr_get_stats": (C.c_int, [C.c_void_p, C.POINTER(KrStats)]),

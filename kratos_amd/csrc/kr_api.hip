@@ -19,20 +19,29 @@
 
 namespace kr {
 
-__global__ void k_cycle_begin(TimeState *ts) {
-    // cycle::step stop rule (mesh/cycle/cycle.cpp:143-144)
+__global__ void k_cycle_begin(TimeState *tsv, int ngrp) {
+    // cycle::step stop rule (mesh/cycle/cycle.cpp:143-144), per time group
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrp) return;
+    TimeState *ts = tsv + g;
     ts->skip = (ts->t >= ts->t_lim) ? 1 : 0;
     if (!ts->skip) ts->cycles += 1;
 }
 
 // Global CFL reduction and the dt / t bookkeeping of redc_dt_start/finish
 // (mesh/cycle/cycle.cpp:77-115): p_dt = min(p_dt, CFL), dt_use clamped to
-// t_lim when topping up, t advanced, next seed = dt_use * dt_expand.
-__global__ void __launch_bounds__(1024) k_dt_finalize(TimeState *ts, const double *part, int npart) {
+// t_lim when topping up, t advanced, next seed = dt_use * dt_expand.  One
+// workgroup per time group; the group's blocks are contiguous, so its CFL
+// partials are part[grp_first[g] * ppb, grp_first[g + 1] * ppb).
+__global__ void __launch_bounds__(1024) k_dt_finalize(TimeState *tsv, const double *part,
+                                                      const int *grp_first, int ppb) {
+    TimeState *ts = tsv + blockIdx.x;
     if (ts->skip) return;
     __shared__ double red[16];
     double v = (double)FLT_MAX;
-    for (int i = threadIdx.x; i < npart; i += blockDim.x) v = pmin(v, part[i]);
+    const size_t i0 = (size_t)grp_first[blockIdx.x] * ppb;
+    const size_t i1 = (size_t)grp_first[blockIdx.x + 1] * ppb;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) v = pmin(v, part[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = pmin(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -85,6 +94,9 @@ struct HipError : std::runtime_error {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// graph replays between host checks of the stop rule in kr_advance
+constexpr int kAdvanceChunk = 64;
+
 }  // namespace
 
 struct kr_ctx {
@@ -103,7 +115,10 @@ struct kr_ctx {
     double *U = nullptr, *BF = nullptr;
     void *W = nullptr, *F[3] = {nullptr, nullptr, nullptr}, *EE = nullptr;
     BlockGeo *geo = nullptr;
-    TimeState *ts = nullptr;
+    TimeState *ts = nullptr;        // [ngrp]
+    int ngrp = 1;
+    int *grp_first = nullptr;       // [ngrp + 1] first internal block of each group
+    int ppb = 0;                    // CFL partials per block
     double *dt_part = nullptr, *red_part = nullptr, *red_out = nullptr;
     size_t n_dt_part = 0, n_red_part = 0;
     hipStream_t stream = nullptr;
@@ -143,8 +158,8 @@ namespace {
 void release(kr_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
-    void *bufs[] = {c->U, c->BF, c->W, c->F[0], c->F[1], c->F[2], c->EE,
-                    c->geo, c->ts, c->dt_part, c->red_part, c->red_out};
+    void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE,
+                    c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->U = c->BF = nullptr;
@@ -152,6 +167,7 @@ void release(kr_ctx *c) {
     c->F[0] = c->F[1] = c->F[2] = nullptr;
     c->geo = nullptr;
     c->ts = nullptr;
+    c->grp_first = nullptr;
     c->dt_part = c->red_part = c->red_out = nullptr;
 }
 
@@ -168,6 +184,19 @@ void commit(kr_ctx *c) {
     if (c->committed) return;
     if (c->descs.empty()) throw std::invalid_argument("no blocks registered");
     const size_t nb = c->descs.size();
+    // internal order: by time group (stable), so each group's blocks and
+    // CFL partials are contiguous
+    std::stable_sort(c->descs.begin(), c->descs.end(),
+                     [](const kr_block_desc &a, const kr_block_desc &b) { return a.group < b.group; });
+    c->id2idx.clear();
+    for (size_t b = 0; b < nb; ++b) c->id2idx[c->descs[b].id] = (int)b;
+    c->ngrp = c->descs.back().group + 1;
+    std::vector<int> first(c->ngrp + 1, 0);
+    for (size_t b = 0; b < nb; ++b) first[c->descs[b].group + 1] += 1;
+    for (int g = 0; g < c->ngrp; ++g) {
+        if (first[g + 1] == 0) throw std::invalid_argument("time group " + std::to_string(g) + " has no blocks");
+        first[g + 1] += first[g];
+    }
     for (int a = 0; a < 3; ++a) c->n[a] = c->cfg.n_cell_block[a];
     c->ndim = 0;
     for (int a = 0; a < 3; ++a) c->ndim += c->n[a] > 1 ? 1 : 0;
@@ -204,6 +233,7 @@ void commit(kr_ctx *c) {
         for (int a = 0; a < 3; ++a) v *= G.dx[a];
         G.vol = v;
         G.phy = 0;
+        G.grp = d.group;
         for (int k = 0; k < 27; ++k) {
             const int id = d.nbr[k];
             if (id < 0) {
@@ -213,6 +243,8 @@ void commit(kr_ctx *c) {
             auto it = c->id2idx.find(id);
             if (it == c->id2idx.end())
                 throw std::invalid_argument("neighbour id " + std::to_string(id) + " not registered");
+            if (c->descs[it->second].group != d.group)
+                throw std::invalid_argument("neighbouring blocks in different time groups");
             G.nbr[k] = it->second;
         }
         for (int a = 0; a < 3; ++a)
@@ -236,10 +268,14 @@ void commit(kr_ctx *c) {
     ts.dt_use = c->cfg.dt_init;
     ts.dt_expand = c->cfg.dt_expand;
     ts.top_up = c->cfg.top_up;
-    c->ts = dalloc<TimeState>(1);
-    KR_HIP(hipMemcpy(c->ts, &ts, sizeof(ts), hipMemcpyHostToDevice));
+    std::vector<TimeState> tsv(c->ngrp, ts);
+    c->ts = dalloc<TimeState>(c->ngrp);
+    KR_HIP(hipMemcpy(c->ts, tsv.data(), c->ngrp * sizeof(ts), hipMemcpyHostToDevice));
+    c->grp_first = dalloc<int>(c->ngrp + 1);
+    KR_HIP(hipMemcpy(c->grp_first, first.data(), (c->ngrp + 1) * sizeof(int), hipMemcpyHostToDevice));
 
-    c->n_dt_part = nb * ceil_div(c->ncell, kThreads);
+    c->ppb = (int)ceil_div(c->ncell, kThreads);
+    c->n_dt_part = nb * c->ppb;
     c->dt_part = dalloc<double>(c->n_dt_part);
     c->n_red_part = c->n_dt_part;
     c->red_part = dalloc<double>(c->n_red_part);
@@ -306,8 +342,9 @@ struct Launcher {
         });
         if (step == 0)
             run(KR_K_DTFIN, [&] {
-                hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1024), 0, c->stream, c->ts,
-                                   (const double *)c->dt_part, (int)c->n_dt_part);
+                hipLaunchKernelGGL(k_dt_finalize, dim3(c->ngrp), dim3(1024), 0, c->stream, c->ts,
+                                   (const double *)c->dt_part, (const int *)c->grp_first,
+                                   c->ppb);
             });
         if constexpr (MHD) {
             if (c->fused) {
@@ -347,7 +384,8 @@ struct Launcher {
 
     template <class FT, bool MHD> void cycle_t() {
         run(KR_K_BEGIN, [&] {
-            hipLaunchKernelGGL(k_cycle_begin, dim3(1), dim3(1), 0, c->stream, c->ts);
+            hipLaunchKernelGGL(k_cycle_begin, dim3(ceil_div(c->ngrp, 256)), dim3(256), 0,
+                               c->stream, c->ts, c->ngrp);
         });
         stage<FT, MHD>(0);
         stage<FT, MHD>(1);
@@ -385,10 +423,29 @@ template <class F> int guarded(kr_ctx *c, F &&f) {
     }
 }
 
-TimeState read_ts(kr_ctx *c) {
+TimeState read_ts(kr_ctx *c, int g = 0) {
     TimeState ts{};
-    KR_HIP(hipMemcpy(&ts, c->ts, sizeof(ts), hipMemcpyDeviceToHost));
+    KR_HIP(hipMemcpy(&ts, c->ts + g, sizeof(ts), hipMemcpyDeviceToHost));
     return ts;
+}
+
+std::vector<TimeState> read_all_ts(kr_ctx *c) {
+    std::vector<TimeState> v(c->ngrp);
+    KR_HIP(hipMemcpy(v.data(), c->ts, c->ngrp * sizeof(TimeState), hipMemcpyDeviceToHost));
+    return v;
+}
+
+// cycles run by the slowest-stopping group between two snapshots, and
+// whether every group has reached t_lim
+int cycles_between(const std::vector<TimeState> &a, const std::vector<TimeState> &b) {
+    int n = 0;
+    for (size_t g = 0; g < a.size(); ++g) n = std::max(n, b[g].cycles - a[g].cycles);
+    return n;
+}
+bool all_done(const std::vector<TimeState> &v) {
+    for (auto &t : v)
+        if (t.t < t.t_lim) return false;
+    return true;
 }
 
 }  // namespace
@@ -450,6 +507,7 @@ int kr_add_blocks(kr_ctx *c, int n, const kr_block_desc *blocks) {
                     throw std::invalid_argument("block size differs from n_cell_block");
             if (d.level != 0 && d.level != blocks[0].level)
                 throw std::invalid_argument("only same-level meshes are supported");
+            if (d.group < 0 || d.group >= (1 << 20)) throw std::invalid_argument("bad time group");
             if (c->id2idx.count(d.id)) throw std::invalid_argument("duplicate block id");
             c->id2idx[d.id] = (int)c->descs.size();
             c->descs.push_back(d);
@@ -536,7 +594,7 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
 int kr_step(kr_ctx *c, double *dt_taken) {
     return guarded(c, [&] {
         commit(c);
-        const TimeState before = read_ts(c);
+        const auto before = read_all_ts(c);
         KR_HIP(hipEventRecord(c->ev0, c->stream));
         Launcher L{c};
         L.cycle();
@@ -544,10 +602,11 @@ int kr_step(kr_ctx *c, double *dt_taken) {
         KR_HIP(hipStreamSynchronize(c->stream));
         float ms = 0;
         KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        const TimeState after = read_ts(c);
+        const auto after = read_all_ts(c);
         c->stats.gpu_ms = ms;
-        c->stats.cycles = after.cycles - before.cycles;
-        if (dt_taken) *dt_taken = c->stats.cycles ? after.dt_use : 0.0;
+        c->stats.cycles = cycles_between(before, after);
+        if (dt_taken)
+            *dt_taken = after[0].cycles != before[0].cycles ? after[0].dt_use : 0.0;
         return KR_OK;
     });
 }
@@ -565,16 +624,27 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
             KR_HIP(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
             KR_HIP(hipGraphDestroy(g));
         }
-        const TimeState before = read_ts(c);
-        KR_HIP(hipEventRecord(c->ev0, c->stream));
-        for (int i = 0; i < n_cycles; ++i) KR_HIP(hipGraphLaunch(c->graph, c->stream));
-        KR_HIP(hipEventRecord(c->ev1, c->stream));
-        KR_HIP(hipStreamSynchronize(c->stream));
-        float ms = 0;
-        KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        const TimeState after = read_ts(c);
-        c->stats.gpu_ms = ms;
-        c->stats.cycles = after.cycles - before.cycles;
+        // replays in chunks; between chunks the host stops early once every
+        // group has reached t_lim (later replays would skip all work)
+        const auto before = read_all_ts(c);
+        double ms_tot = 0;
+        int left = n_cycles;
+        bool done = all_done(before);
+        while (left > 0 && !done) {
+            const int chunk = left < kAdvanceChunk ? left : kAdvanceChunk;
+            KR_HIP(hipEventRecord(c->ev0, c->stream));
+            for (int i = 0; i < chunk; ++i) KR_HIP(hipGraphLaunch(c->graph, c->stream));
+            KR_HIP(hipEventRecord(c->ev1, c->stream));
+            KR_HIP(hipStreamSynchronize(c->stream));
+            float ms = 0;
+            KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            ms_tot += ms;
+            left -= chunk;
+            if (left > 0) done = all_done(read_all_ts(c));
+        }
+        const auto after = read_all_ts(c);
+        c->stats.gpu_ms = ms_tot;
+        c->stats.cycles = cycles_between(before, after);
         if (n_done) *n_done = c->stats.cycles;
         return KR_OK;
     });
@@ -620,10 +690,25 @@ int kr_set_time(kr_ctx *c, double t, double dt_seed) {
     return guarded(c, [&] {
         commit(c);
         KR_HIP(hipStreamSynchronize(c->stream));
-        TimeState ts = read_ts(c);
-        ts.t = t;
-        ts.p_dt = dt_seed < FLT_MIN ? (double)(FLT_MAX / 10) : dt_seed;
-        KR_HIP(hipMemcpy(c->ts, &ts, sizeof(ts), hipMemcpyHostToDevice));
+        auto v = read_all_ts(c);
+        for (auto &ts : v) {
+            ts.t = t;
+            ts.p_dt = dt_seed < FLT_MIN ? (double)(FLT_MAX / 10) : dt_seed;
+        }
+        KR_HIP(hipMemcpy(c->ts, v.data(), v.size() * sizeof(TimeState), hipMemcpyHostToDevice));
+        return KR_OK;
+    });
+}
+
+int kr_get_group_time(kr_ctx *c, int group, double *t, double *dt_last, int *cycles) {
+    return guarded(c, [&] {
+        commit(c);
+        if (group < 0 || group >= c->ngrp) throw std::invalid_argument("no such time group");
+        KR_HIP(hipStreamSynchronize(c->stream));
+        const TimeState ts = read_ts(c, group);
+        if (t) *t = ts.t;
+        if (dt_last) *dt_last = ts.dt_use;
+        if (cycles) *cycles = ts.cycles;
         return KR_OK;
     });
 }

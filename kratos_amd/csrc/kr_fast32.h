@@ -1,0 +1,178 @@
+// kr_fast32.h -- FP32 flux functions of the mixed precision form.
+//
+// The mixed form is not bound to the reference's rounding (its bar is an L1
+// distance to the FP64 oracle), so the FP32 Riemann solve and PLM use the
+// hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32, 1 ulp) instead
+// of the IEEE-exact division sequences, and the HLLD solution is evaluated
+// branch-free: every face computes the L/R star states and the double-star
+// state of its side, and the region (supersonic, star, double star) only
+// selects the weights.  Wave speeds and intermediate states are those of
+// hlld<FT> (mhd/riemann/hlld.h:81-298, Miyoshi & Kusano 2005).
+// Included by kr_physics.h after its variable indices (RHO, ENE, MOM, BCC).
+#pragma once
+
+namespace kr {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsq(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Harmonic (van Leer) PLM increment of one cell: dl*dr/(dl+dr) where the
+// differences agree in sign, else 0 (hydro/reconstruct/plm.h:116-153).
+__device__ __forceinline__ float plm_inc_fast(float qm, float q0, float qp) {
+    const float dl = q0 - qm, dr = qp - q0;
+    const float pr = dl * dr;
+    return pr > 0.f ? pr * frcp(dl + dr) : 0.f;
+}
+
+// Face states of the face between cells l and r from the four cells
+// ll, l, r, rr (the per-face form of plm<float>).
+__device__ __forceinline__ void plm_fast(float q_ll, float q_l, float q_r, float q_rr, float &ql,
+                                         float &qr) {
+    ql = q_l + plm_inc_fast(q_ll, q_l, q_r);
+    qr = q_r - plm_inc_fast(q_l, q_r, q_rr);
+}
+
+struct Side32 {
+    float r, p, u, v, w, by, bz;   // primitives (rotated frame)
+    float e, pt;                   // total energy, total pressure
+    float cf;                      // fast magnetosonic speed
+};
+
+__device__ __forceinline__ Side32 side32(const float *q, float bx2, float gam, float gm1i) {
+    Side32 s;
+    s.r = q[RHO];
+    s.p = q[ENE];
+    s.u = q[MOM];
+    s.v = q[MOM + 1];
+    s.w = q[MOM + 2];
+    s.by = q[BCC + 1];
+    s.bz = q[BCC + 2];
+    const float ir = frcp(s.r);
+    const float bt2 = s.by * s.by + s.bz * s.bz;
+    const float pm = 0.5f * (bx2 + bt2);
+    s.e = s.p * gm1i + 0.5f * s.r * (s.u * s.u + s.v * s.v + s.w * s.w) + pm;
+    s.pt = s.p + pm;
+    const float cs2 = gam * s.p * ir, va2 = bx2 * ir, vt2 = bt2 * ir;
+    const float q2 = va2 + vt2 + cs2, r2 = va2 + vt2 - cs2;
+    s.cf = fsq(0.5f * (q2 + fsq(r2 * r2 + 4.f * cs2 * vt2)));
+    return s;
+}
+
+// Star state of one side: returns (rho*, my*, mz*, by*, bz*, E*) and u.b*.
+struct Star32 {
+    float r, my, mz, by, bz, e, udb;
+};
+
+__device__ __forceinline__ Star32 star32(const Side32 &s, float bx, float bx2, float sd,
+                                         float smd, float rs, float sm, float pts, float tiny) {
+    Star32 o;
+    o.r = rs;
+    const float rsd = s.r * sd;
+    const float denom = rsd * smd - bx2;
+    const bool deg = fabsf(denom) < tiny * pts;
+    const float idn = frcp(denom);
+    const float tv = bx * (sd - smd) * idn;
+    const float tb = (rsd * sd - bx2) * idn;
+    o.my = deg ? rs * s.v : rs * (s.v - s.by * tv);
+    o.mz = deg ? rs * s.w : rs * (s.w - s.bz * tv);
+    o.by = deg ? s.by : s.by * tb;
+    o.bz = deg ? s.bz : s.bz * tb;
+    const float irs = frcp(rs);
+    o.udb = (rs * sm * bx + o.my * o.by + o.mz * o.bz) * irs;
+    const float wdb = s.u * bx + s.v * s.by + s.w * s.bz;
+    o.e = (bx * (wdb - o.udb) - s.pt * s.u + sd * s.e + pts * sm) * frcp(smd);
+    return o;
+}
+
+// FP32 HLLD flux in the rotated frame (wl / wr: rho, p, v_n, v_t1, v_t2,
+// B_n, B_t1, B_t2).  flx slots as in hlld<FT>: mass, energy, momentum,
+// B_n (0), B_t1, B_t2.
+__device__ __forceinline__ void hlld_fast(float *flx, const float *wl, const float *wr, float gam) {
+    const float gm1i = frcp(gam - 1.f);
+    const float bx = wl[BCC], bx2 = bx * bx;
+    const Side32 L = side32(wl, bx2, gam, gm1i);
+    const Side32 R = side32(wr, bx2, gam, gm1i);
+    const float sl = fminf(L.u - L.cf, R.u - R.cf);
+    const float sr = fmaxf(L.u + L.cf, R.u + R.cf);
+    const float sdl = sl - L.u, sdr = sr - R.u;
+    const float rsdl = L.r * sdl, rsdr = R.r * sdr;
+    const float sm = (rsdr * R.u - R.pt - rsdl * L.u + L.pt) * frcp(rsdr - rsdl);
+    const float sml = sl - sm, smr = sr - sm;
+    const float rsl = rsdl * frcp(sml), rsr = rsdr * frcp(smr);
+    const float isql = frsq(rsl), isqr = frsq(rsr);
+    const float sql = rsl * isql, sqr = rsr * isqr;
+    const float abx = fabsf(bx);
+    const float ssl = sm - abx * isql, ssr = sm + abx * isqr;
+    const float tiny = 1.0e-4f;
+    const float pts = L.pt + rsdl * (sdl - sml);
+    const Star32 SL = star32(L, bx, bx2, sdl, sml, rsl, sm, pts, tiny);
+    const Star32 SR = star32(R, bx, bx2, sdr, smr, rsr, sm, pts, tiny);
+
+    const bool left = sm >= 0.f;
+    // double-star state of the chosen side (equal to the star state when
+    // B_n^2 is below tiny * p*_T)
+    const float sgn = bx > 0.f ? 1.f : -1.f;
+    const float inv = frcp(sql + sqr);
+    const float vy = inv * (SL.my * isql + SR.my * isqr + sgn * (SR.by - SL.by));
+    const float vz = inv * (SL.mz * isql + SR.mz * isqr + sgn * (SR.bz - SL.bz));
+    const float byss = inv * (sql * SR.by + sqr * SL.by +
+                              sgn * sql * sqr * (SR.my * frcp(rsr) - SL.my * frcp(rsl)));
+    const float bzss = inv * (sql * SR.bz + sqr * SL.bz +
+                              sgn * sql * sqr * (SR.mz * frcp(rsr) - SL.mz * frcp(rsl)));
+    const float tmp = sm * bx + (vy * byss + vz * bzss);
+    const bool dstar = bx2 > pts * tiny;
+
+    const Side32 &X = left ? L : R;
+    const Star32 &S = left ? SL : SR;
+    const float sqx = left ? sql : sqr;
+    const float rsx = left ? rsl : rsr;
+    const float s1 = left ? sl : sr;
+    const float s2 = left ? ssl : ssr;
+    const float ess = left ? S.e - sqx * sgn * (S.udb - tmp) : S.e + sqx * sgn * (S.udb - tmp);
+    // double-star conserved (falls back to the star state)
+    const float mys = dstar ? rsx * vy : S.my;
+    const float mzs = dstar ? rsx * vz : S.mz;
+    const float bys = dstar ? byss : S.by;
+    const float bzs = dstar ? bzss : S.bz;
+    const float es = dstar ? ess : S.e;
+    // weights: F = F_X + a (U*_X - U_X) + c (U**_X - U*_X)
+    const bool super_ = sl >= 0.f || sr <= 0.f;
+    float a = left ? (s1 < 0.f ? s1 : 0.f) : (s1 > 0.f ? s1 : 0.f);
+    float c = left ? (s2 < 0.f ? s2 : 0.f) : (s2 > 0.f ? s2 : 0.f);
+    // physical flux of side X
+    const float vbx = X.u * bx + X.v * X.by + X.w * X.bz;
+    const float mx = X.r * X.u;
+    float f[7];
+    f[0] = mx;
+    f[1] = X.u * (X.e + X.pt) - bx * vbx;
+    f[2] = mx * X.u + X.pt - bx2;
+    f[3] = mx * X.v - bx * X.by;
+    f[4] = mx * X.w - bx * X.bz;
+    f[5] = X.by * X.u - bx * X.v;
+    f[6] = X.bz * X.u - bx * X.w;
+    // U_X, U*_X, U**_X
+    const float ux[7] = {X.r, X.e, mx, X.r * X.v, X.r * X.w, X.by, X.bz};
+    const float usx[7] = {rsx, S.e, rsx * sm, S.my, S.mz, S.by, S.bz};
+    const float ussx[7] = {rsx, es, rsx * sm, mys, mzs, bys, bzs};
+    if (super_) {
+        a = 0.f;
+        c = 0.f;
+    }
+    float g[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const float v = f[k] + a * (usx[k] - ux[k]) + c * (ussx[k] - usx[k]);
+        g[k] = super_ ? f[k] : v;
+    }
+    flx[RHO] = g[0];
+    flx[ENE] = g[1];
+    flx[MOM] = g[2];
+    flx[MOM + 1] = g[3];
+    flx[MOM + 2] = g[4];
+    flx[BCC] = 0.f;
+    flx[BCC + 1] = g[5];
+    flx[BCC + 2] = g[6];
+}
+
+}  // namespace kr

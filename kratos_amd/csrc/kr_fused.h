@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
     static_assert(S::NCOL <= NT, "one halo column per thread");
     static_assert(S::NZI <= NT, "one z pencil per thread");
     static_assert(TX * TY + TX + TY <= NT, "one owned cell / edge face per thread");
-    if (m.ts->skip) return;
+    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     const int tid = threadIdx.x;
     const int b = blockIdx.y;
     const int ntx = m.n[0] / TX;
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
     const int nz = m.n[2];
     const bool xlast = x0 + TX == m.n[0], ylast = y0 + TY == m.n[1];
     const BlockGeo &g = m.geo[b];
-    const double dt = m.ts->dt_use * (step == 0 ? 1.0 : 0.5);
+    const double dt = m.ts[g.grp].dt_use * (step == 0 ? 1.0 : 0.5);
     const FT gam = (FT)m.gamma;
     const size_t nc = m.ncell;
 
@@ -237,11 +237,17 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
                         qLn[v] = 0;
                         continue;
                     }
-                    bool flat;
                     const FT q0 = sW[s0][v][c];
-                    const FT t = plm_inc<FT>(sW[sm][v][c], q0, sW[sp][v][c], flat);
-                    qR[v] = q0 + (flat ? (FT)0 : -t);
-                    qLn[v] = q0 + (flat ? (FT)0 : t);
+                    if constexpr (std::is_same_v<FT, float>) {
+                        const float t = plm_inc_fast(sW[sm][v][c], q0, sW[sp][v][c]);
+                        qR[v] = q0 - t;
+                        qLn[v] = q0 + t;
+                    } else {
+                        bool flat;
+                        const FT t = plm_inc<FT>(sW[sm][v][c], q0, sW[sp][v][c], flat);
+                        qR[v] = q0 + (flat ? (FT)0 : -t);
+                        qLn[v] = q0 + (flat ? (FT)0 : t);
+                    }
                 }
                 if (p >= 0 && p <= nz) {
                     FT wl[8], wr[8], flx[8];
@@ -257,7 +263,10 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
                     rot<2>(wr + MOM);
                     rot<2>(wl + BCC);
                     rot<2>(wr + BCC);
-                    hlld<FT>(flx, wl, wr, gam);
+                    if constexpr (std::is_same_v<FT, float>)
+                        hlld_fast(flx, wl, wr, gam);
+                    else
+                        hlld<FT>(flx, wl, wr, gam);
                     rot<1>(flx + MOM);
                     // EMF rewrite for axis 2: E_x = flx[BCC+2], E_y = -flx[BCC+1]
                     const int gz = S::zg(zx, zy);
@@ -317,18 +326,25 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
                     if (k >= BCC) v = BCC + (a + k - BCC) % 3;
                     if (k == BCC) continue;            // normal field: face value
                     const FT *q = &sW[s0][v][0];
-                    bool fl, fr;
                     const FT qm2 = q[cR - 2 * stride], qm1 = q[cR - stride];
                     const FT q00 = q[cR], qp1 = q[cR + stride];
-                    const FT tl = plm_inc<FT>(qm2, qm1, q00, fl);
-                    const FT tr = plm_inc<FT>(qm1, q00, qp1, fr);
-                    wl[k] = qm1 + (fl ? (FT)0 : tl);
-                    wr[k] = q00 + (fr ? (FT)0 : -tr);
+                    if constexpr (std::is_same_v<FT, float>) {
+                        plm_fast(qm2, qm1, q00, qp1, wl[k], wr[k]);
+                    } else {
+                        bool fl, fr;
+                        const FT tl = plm_inc<FT>(qm2, qm1, q00, fl);
+                        const FT tr = plm_inc<FT>(qm1, q00, qp1, fr);
+                        wl[k] = qm1 + (fl ? (FT)0 : tl);
+                        wr[k] = q00 + (fr ? (FT)0 : -tr);
+                    }
                 }
                 const FT bn = bfn(a, fx, fy, p);
                 wl[BCC] = bn;
                 wr[BCC] = bn;
-                hlld<FT>(flx, wl, wr, gam);
+                if constexpr (std::is_same_v<FT, float>)
+                    hlld_fast(flx, wl, wr, gam);
+                else
+                    hlld<FT>(flx, wl, wr, gam);
                 // back to the mesh frame
                 sFh[0][it] = flx[RHO];
                 sFh[1][it] = flx[ENE];

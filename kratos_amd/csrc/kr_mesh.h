@@ -18,12 +18,14 @@ struct BlockGeo {
     double vol;        // vol0
     int nbr[27];       // block index at ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical side
     int phy;           // bit (2*axis + side) set on a physical side
-    int pad;
+    int grp;           // time group: index of this block's TimeState
 };
 
 // Time / dt bookkeeping kept on the device so that a whole cycle (and a
 // multi-cycle hipGraph) runs without host round trips.  Mirrors
-// mesh/cycle/cycle.cpp:77-115 and :124-150.
+// mesh/cycle/cycle.cpp:77-115 and :124-150.  One per time group: a group is
+// one mesh with its own t and dt (a single mesh is group 0; an ensemble of
+// independent boxes is one group per box, all advanced by the same launches).
 struct TimeState {
     double t;          // simulation time
     double t_lim;
@@ -54,8 +56,8 @@ struct MeshView {
     FT *F[3];          // [block][var nv][nfa[a]]
     FT *EE;            // [block][comp 3][nface]                   (MHD)
     const BlockGeo *geo;
-    TimeState *ts;
-    double *dt_part;   // one CFL partial per c2p workgroup
+    TimeState *ts;     // [group]
+    double *dt_part;   // one CFL partial per c2p workgroup, block-major
 };
 
 }  // namespace kr

@@ -407,6 +407,12 @@ __device__ __forceinline__ void hlld(FT *flx, const FT *wli, const FT *wri, FT g
     }
 }
 
+}  // namespace kr
+
+#include "kr_fast32.h"
+
+namespace kr {
+
 // Face flux in the mesh frame for a sweep along `axis`: hllc.h:32-43 and
 // hlld.h:29-47 (the B-flux slots are rewritten as edge EMFs, normal slot 0).
 // ql / qr are rotated in place.
@@ -428,7 +434,10 @@ __device__ __forceinline__ void face_flux(FT *ql, FT *qr, FT *flx, FT gam) {
     if constexpr (MHD) {
         rot<AXIS>(ql + BCC);
         rot<AXIS>(qr + BCC);
-        hlld<FT>(flx, ql, qr, gam);
+        if constexpr (std::is_same_v<FT, float>)
+            hlld_fast(flx, ql, qr, gam);
+        else
+            hlld<FT>(flx, ql, qr, gam);
         rot<3 - AXIS>(flx + MOM);
         const FT e0 = flx[BCC + 2], e1 = -flx[BCC + 1];
         flx[BCC + (AXIS + 1) % 3] = e0;

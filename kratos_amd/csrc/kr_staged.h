@@ -146,7 +146,7 @@ __device__ __forceinline__ double block_min(double v) {
 // face average, and at substep 0 the per-cell CFL dt reduced per workgroup.
 template <class FT, bool MHD>
 __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
-    if (m.ts->skip) return;
+    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
     const int b = blockIdx.y;
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
 template <class FT, bool MHD, int AXIS>
 __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
     constexpr int axis = AXIS;
-    if (m.ts->skip) return;
+    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
     const int b = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,7 +250,10 @@ __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         if (MHD && v == BCC + axis) continue;
-        plm<FT>(q[0][v], q[1][v], q[2][v], q[3][v], ql[v], qr[v]);
+        if constexpr (std::is_same_v<FT, float>)
+            plm_fast(q[0][v], q[1][v], q[2][v], q[3][v], ql[v], qr[v]);
+        else
+            plm<FT>(q[0][v], q[1][v], q[2][v], q[3][v], ql[v], qr[v]);
     }
     if constexpr (MHD) {
         const FT bn = (FT)load_bf(m, b, step, axis, p);
@@ -281,7 +284,7 @@ __device__ __forceinline__ FT flux_at(const MeshView<FT> &m, int b, int a, const
 // from the neighbour's primitives (:201-213); inside it equals the c2p ec.
 template <class FT>
 __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
-    if (m.ts->skip) return;
+    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     const int b = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.nface) return;
@@ -336,7 +339,7 @@ __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
 // (mhd/integrator/integrator.h:385-462 with rk2.h:191-205).
 template <class FT, bool MHD>
 __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
-    if (m.ts->skip) return;
+    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
     const int b = blockIdx.y;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
     i[0] = (int)(t % e0);
     i[1] = (int)((t / e0) % e1);
     i[2] = (int)(t / ((size_t)e0 * e1));
-    const double dt = m.ts->dt_use * (step == 0 ? 1.0 : 0.5);
+    const double dt = m.ts[m.geo[b].grp].dt_use * (step == 0 ? 1.0 : 0.5);
     const BlockGeo &g = m.geo[b];
     if (i[0] < m.n[0] && i[1] < m.n[1] && i[2] < m.n[2]) {
         const size_t c = cidx(m.n, i[0], i[1], i[2]);

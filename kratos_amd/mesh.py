@@ -88,22 +88,30 @@ class MeshSpec:
                     out[k] = self.bid(q) if ok else -1
         return out
 
-    def block_descs(self):
-        arr = (_capi.KrBlockDesc * self.n_blocks)()
-        for b in range(self.n_blocks):
-            d = arr[b]
-            d.id = b
-            d.level = 0
-            r = self.reg(b)
-            xf0, dx0 = self.geometry(b)
-            for a in range(3):
-                d.reg[a] = r[a]
-                d.n_ceff[a] = self.n_cell_block[a]
-                d.xf0[a] = xf0[a]
-                d.dx0[a] = dx0[a]
-            for k, v in enumerate(self.neighbours(b)):
-                d.nbr[k] = v
+    def block_descs(self, members=1):
+        """Descriptors of `members` independent copies of this mesh: copy m
+        has block ids m * n_blocks + b and its own time group m."""
+        nb = self.n_blocks
+        arr = (_capi.KrBlockDesc * (nb * members))()
+        for m in range(members):
+            for b in range(nb):
+                self._desc(arr[m * nb + b], b, m)
         return arr
+
+    def _desc(self, d, b, m):
+        nb = self.n_blocks
+        d.id = m * nb + b
+        d.level = 0
+        d.group = m
+        r = self.reg(b)
+        xf0, dx0 = self.geometry(b)
+        for a in range(3):
+            d.reg[a] = r[a]
+            d.n_ceff[a] = self.n_cell_block[a]
+            d.xf0[a] = xf0[a]
+            d.dx0[a] = dx0[a]
+        for k, v in enumerate(self.neighbours(b)):
+            d.nbr[k] = v + m * nb if v >= 0 else -1
 
     # ---------------------------------------------------- block <-> global
     def block_slices(self, b):
@@ -189,9 +197,14 @@ class Context:
     """One device context of the C-ABI over a uniform block mesh."""
 
     def __init__(self, spec: MeshSpec, phys: Physics, precision="fp64", path="auto",
-                 tile=(0, 0)):
+                 tile=(0, 0), members=1):
+        """`members` > 1 makes an ensemble: that many independent copies of
+        the mesh in one context, each its own time group (own t and dt),
+        all advanced by the same launches.  upload / download / time take
+        the member index."""
         lib = _capi.load()
         self.lib, self.spec, self.phys = lib, spec, phys
+        self.members = int(members)
         self.mhd = phys.mhd
         cfg = _capi.KrConfig()
         cfg.model = _capi.KR_MHD if phys.mhd else _capi.KR_HYDRO
@@ -211,8 +224,9 @@ class Context:
         rc = lib.kr_create(C.byref(cfg), C.byref(h))
         _capi.check(rc, None, "kr_create")
         self.h = h
-        descs = spec.block_descs()
-        _capi.check(lib.kr_add_blocks(h, spec.n_blocks, descs), h, "kr_add_blocks")
+        descs = spec.block_descs(self.members)
+        _capi.check(lib.kr_add_blocks(h, spec.n_blocks * self.members, descs), h,
+                    "kr_add_blocks")
 
     def close(self):
         if self.h:
@@ -225,23 +239,26 @@ class Context:
         except Exception:
             pass
 
-    def upload(self, u, bfs=None):
+    def upload(self, u, bfs=None, member=0):
+        off = member * self.spec.n_blocks
         for b, (ua, ba) in enumerate(self.spec.split(u, bfs if self.mhd else None)):
             ua = np.ascontiguousarray(ua, dtype=np.float64)
             bp = None
             if ba is not None:
                 ba = np.ascontiguousarray(ba, dtype=np.float64)
                 bp = ba.ctypes.data
-            _capi.check(self.lib.kr_upload(self.h, b, ua.ctypes.data, bp), self.h, "kr_upload")
+            _capi.check(self.lib.kr_upload(self.h, off + b, ua.ctypes.data, bp), self.h,
+                        "kr_upload")
 
-    def download(self):
+    def download(self, member=0):
+        off = member * self.spec.n_blocks
         n = self.spec.n_cell_block
         nvar = 8 if self.mhd else 5
         blocks = []
         for b in range(self.spec.n_blocks):
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
             ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
-            _capi.check(self.lib.kr_download(self.h, b, ua.ctypes.data,
+            _capi.check(self.lib.kr_download(self.h, off + b, ua.ctypes.data,
                                              ba.ctypes.data if ba is not None else None),
                         self.h, "kr_download")
             blocks.append((ua, ba))
@@ -258,7 +275,14 @@ class Context:
                     "kr_advance")
         return done.value
 
-    def time(self):
+    def time(self, member=None):
+        """(t, dt_last, dt_seed) of the mesh, or (t, dt_last, cycles) of
+        ensemble member `member`."""
+        if member is not None:
+            t, dt, cyc = C.c_double(), C.c_double(), C.c_int()
+            _capi.check(self.lib.kr_get_group_time(self.h, int(member), C.byref(t), C.byref(dt),
+                                                   C.byref(cyc)), self.h, "kr_get_group_time")
+            return t.value, dt.value, cyc.value
         t, dt, seed = C.c_double(), C.c_double(), C.c_double()
         _capi.check(self.lib.kr_get_time(self.h, C.byref(t), C.byref(dt), C.byref(seed)),
                     self.h, "kr_get_time")

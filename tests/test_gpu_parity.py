@@ -101,3 +101,35 @@ def test_hydro_3d_blocks_reflect(gpu_lib):
     u, _, _ = problems.turbulence(N, seed=2)
     (gu, _, gt), (ou, _, ot) = run_both(spec, phys, u, None, 8)
     np.testing.assert_array_equal(gu, ou)
+
+
+@pytest.mark.parametrize("path", ["staged", "fused"])
+def test_ensemble_per_box_dt(gpu_lib, path):
+    """Independent periodic boxes in one context, each its own time group:
+    every box takes its own dt (different seeds and beta) and stops at t_lim
+    on its own; each must equal an oracle run of that box alone, bit for bit."""
+    N = (16, 16, 16)
+    spec = kratos_amd.MeshSpec(N, N)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3, t_lim=0.02, top_up=True)
+    betas = [0.1, 1.0, 3.0, 10.0]
+    ics = [problems.turbulence(N, seed=s, beta=bt)[:2] for s, bt in enumerate(betas)]
+    print(f"\n[case] ensemble {len(ics)} x {N} path={path}", flush=True)
+    ctx = kratos_amd.Context(spec, phys, "fp64", path=path, members=len(ics))
+    for m, (u, b) in enumerate(ics):
+        ctx.upload(u, b, member=m)
+    ctx.advance(100000)
+    cycles = []
+    for m, (u, b) in enumerate(ics):
+        gu, gb = ctx.download(member=m)
+        t, dt, cyc = ctx.time(member=m)
+        cycles.append(cyc)
+        ref = oracle.OracleMesh(spec, phys)
+        ref.upload(u, b)
+        ref.advance(100000)
+        ou, ob = ref.download()
+        assert t == ref.time()[0] == 0.02
+        np.testing.assert_array_equal(gu, ou)
+        for x, y in zip(gb, ob):
+            np.testing.assert_array_equal(x, y)
+    ctx.close()
+    assert len(set(cycles)) > 1, "boxes should need different cycle counts"
