@@ -97,7 +97,8 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
     st = ctx.stats()
     prof = ctx.profile(profile_cycles) if profile_cycles else None
     return ctx, {"wall": wall, "cycles": done, "gpu_ms": st["gpu_ms"], "prof": prof,
-                 "n_cells": st["n_cells"]}
+                 "n_cells": st["n_cells"], "sweep_tile": st["sweep_tile"],
+                 "tune_ms": [round(t, 4) for t in st["tune_ms"]]}
 
 
 def accuracy_leg(n, nb, beta, t_lim, path):
@@ -182,6 +183,18 @@ def cpu_baseline(seconds_target=15.0, threads=None):
                       f"{wall:.1f} s, FP64 C++ oracle on {threads} threads"}
 
 
+def copy_probe():
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch.mul(a, 1.0, out=b)", "bytes_read_per_launch": 4 * n,
+                      "bytes_written_per_launch": 4 * n, "launches": 5}))
+
+
 def committed_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -213,7 +226,13 @@ def main():
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-n", type=int, default=64)
     ap.add_argument("--t-ens", type=float, default=0.05)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB fp32 copy kernel (calibrates the HBM "
+                         "byte counters of rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
@@ -267,7 +286,9 @@ def main():
             units = nc * (nb + 1) * (nb + 2) ** 2 / nb ** 3   # faces incl. transverse ghost lines
         bytes_launch = alg_bytes(dom, fs, fused) * units
         achieved = bytes_launch / (kms[dom] * 1e-3) / 1e9
-        kname = ("k_sweep (fused stage)" if fused and dom == "update" else f"k_{dom}")
+        tl = r["sweep_tile"]
+        kname = (f"k_sweep<{'float' if head == 'mixed' else 'double'},{tl[0]},{tl[1]},{tl[2]},{tl[3]}>"
+                 if fused and dom == "update" else f"k_{dom}")
         traffic = committed_traffic(kname, workload)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -305,6 +326,7 @@ def main():
         "forms": {f: {"cell_updates_per_s": v["rate"],
                       "ms_per_cycle": v["wall"] / max(v["cycles"], 1) * 1e3,
                       "divb_max": v["divb"],
+                      "sweep_tile": v["sweep_tile"], "autotune_ms": v["tune_ms"],
                       "kernel_ms": v["prof"]["kernel_ms"] if v["prof"] else None}
                   for f, v in res.items()},
         "speedup_mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]

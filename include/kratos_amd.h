@@ -46,7 +46,10 @@ typedef struct kr_config {
     int    n_cell_block[3];
     int    kernel_path;    /* kr_path: KR_PATH_AUTO picks the fused MHD stage
                               kernel when the block shape allows it         */
-    int    tile[2];        /* fused-kernel column tile (x, y); 0 = autotune  */
+    int    tile[3];        /* fused-kernel column tile (x, y) and minimum
+                              waves per SIMD; 0 entries are left to the
+                              autotuner, which times every fitting variant
+                              at the first step                             */
 } kr_config;
 
 enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };
@@ -73,6 +76,10 @@ typedef struct kr_stats {
     double kernel_ms[8];   /* mean launch time per kernel kind (kr_profile)  */
     long long kernel_calls[8];
     double alg_bytes_per_cell_update;  /* algorithmic HBM bytes            */
+    int    sweep_tile[4];  /* fused kernel in use: tile x, y, threads, waves
+                              per SIMD asked (zeros: staged path)           */
+    float  tune_ms[8];     /* autotuner: mean substep-0 sweep time per
+                              variant (0: not timed)                        */
 } kr_stats;
 
 enum kr_kernel_kind {

@@ -32,7 +32,7 @@ class KrConfig(C.Structure):
         ("top_up", C.c_int),
         ("n_cell_block", C.c_int * 3),
         ("kernel_path", C.c_int),
-        ("tile", C.c_int * 2),
+        ("tile", C.c_int * 3),
     ]
 
 
@@ -58,6 +58,8 @@ class KrStats(C.Structure):
         ("kernel_ms", C.c_double * 8),
         ("kernel_calls", C.c_longlong * 8),
         ("alg_bytes_per_cell_update", C.c_double),
+        ("sweep_tile", C.c_int * 4),
+        ("tune_ms", C.c_float * 8),
     ]
 
 

@@ -97,6 +97,19 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 // graph replays between host checks of the stop rule in kr_advance
 constexpr int kAdvanceChunk = 64;
 
+// Fused stage-kernel variants: column tile, workgroup size, and the minimum
+// waves per SIMD asked of the compiler (1 = no constraint).  The autotuner
+// times every variant that fits the block shape (waves > 1 only for the
+// FP32 flux form) and keeps the fastest.
+struct SweepVariant {
+    int tx, ty, nt, waves;
+};
+constexpr SweepVariant kSweepVariants[] = {
+    {32, 8, 512, 1}, {32, 8, 512, 4}, {16, 16, 512, 1}, {16, 16, 512, 4},
+    {16, 8, 256, 1}, {8, 8, 192, 1},
+};
+constexpr int kNumSweepVariants = sizeof(kSweepVariants) / sizeof(kSweepVariants[0]);
+
 }  // namespace
 
 struct kr_ctx {
@@ -107,7 +120,9 @@ struct kr_ctx {
     bool committed = false;
     bool mhd = false, mixed = false;
     bool fused = false;
-    int tile[2] = {0, 0};
+    int variant = -1;               // kSweepVariants index of the fused kernel
+    std::vector<int> candidates;    // variants left for the autotuner
+    bool tuned = false;
     int n[3] = {0, 0, 0};
     int ndim = 0, nv = 5;
     size_t ncell = 0, nface = 0, nfa[3] = {0, 0, 0};
@@ -125,6 +140,7 @@ struct kr_ctx {
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     kr_stats stats{};
+    float tune_ms[16] = {0};
     std::vector<double> stage;   // host staging for packing
 
     template <class FT> MeshView<FT> view() const {
@@ -281,22 +297,24 @@ void commit(kr_ctx *c) {
     c->red_part = dalloc<double>(c->n_red_part);
     c->red_out = dalloc<double>(1);
     // fused stage kernel: 3-D MHD blocks divisible by a supported tile
+    // (kr_config.tile forces a shape); the variant is picked by timing at
+    // the first step
     if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
-        static const int shapes[][2] = {{32, 8}, {16, 8}, {8, 8}};
-        int tx = c->cfg.tile[0], ty = c->cfg.tile[1];
-        bool ok = false;
-        for (auto &sh : shapes) {
-            if (tx && (sh[0] != tx || sh[1] != ty)) continue;
-            if (c->n[0] % sh[0] == 0 && c->n[1] % sh[1] == 0 && c->n[2] >= 4) {
-                c->tile[0] = sh[0];
-                c->tile[1] = sh[1];
-                ok = true;
-                break;
-            }
+        const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
+        c->candidates.clear();
+        for (int v = 0; v < kNumSweepVariants; ++v) {
+            const SweepVariant &sv = kSweepVariants[v];
+            if (tx && (sv.tx != tx || sv.ty != ty)) continue;
+            if (tw && sv.waves != tw) continue;
+            if (sv.waves > 1 && !c->mixed) continue;
+            if (c->n[0] % sv.tx == 0 && c->n[1] % sv.ty == 0 && c->n[2] >= 4)
+                c->candidates.push_back(v);
         }
-        if (!ok && c->cfg.kernel_path == KR_PATH_FUSED)
+        if (c->candidates.empty() && c->cfg.kernel_path == KR_PATH_FUSED)
             throw std::invalid_argument("no fused tile fits the block shape");
-        c->fused = ok;
+        c->fused = !c->candidates.empty();
+        if (c->fused) c->variant = c->candidates[0];
+        c->tuned = c->candidates.size() <= 1;
     }
     c->stats.n_blocks = (int)nb;
     c->stats.n_cells = (long long)(nb * c->ncell);
@@ -325,11 +343,24 @@ struct Launcher {
         evs.push_back({kind, {a, b}});
     }
 
-    template <class FT, int TX, int TY, int NT> void sweep(const MeshView<FT> &m, int step) {
+    template <class FT, int TX, int TY, int NT, int W>
+    void sweep_t(const MeshView<FT> &m, int step) {
         const unsigned nb = (unsigned)c->descs.size();
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
-        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT>), dim3(nt, nb), dim3(NT), 0, c->stream, m,
-                           step);
+        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W>), dim3(nt, nb), dim3(NT), 0, c->stream,
+                           m, step);
+    }
+
+    template <class FT> void sweep(const MeshView<FT> &m, int step, int variant) {
+        constexpr bool f32 = std::is_same_v<FT, float>;
+        switch (variant) {
+        case 0: sweep_t<FT, 32, 8, 512, 1>(m, step); break;
+        case 1: sweep_t<FT, 32, 8, 512, f32 ? 4 : 1>(m, step); break;
+        case 2: sweep_t<FT, 16, 16, 512, 1>(m, step); break;
+        case 3: sweep_t<FT, 16, 16, 512, f32 ? 4 : 1>(m, step); break;
+        case 4: sweep_t<FT, 16, 8, 256, 1>(m, step); break;
+        default: sweep_t<FT, 8, 8, 192, 1>(m, step); break;
+        }
     }
 
     template <class FT, bool MHD> void stage(int step) {
@@ -348,11 +379,7 @@ struct Launcher {
             });
         if constexpr (MHD) {
             if (c->fused) {
-                run(KR_K_UPDATE, [&] {
-                    if (c->tile[0] == 32) sweep<FT, 32, 8, 512>(m, step);
-                    else if (c->tile[0] == 16) sweep<FT, 16, 8, 256>(m, step);
-                    else sweep<FT, 8, 8, 192>(m, step);
-                });
+                run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant); });
                 return;
             }
         }
@@ -402,6 +429,59 @@ struct Launcher {
         KR_HIP(hipGetLastError());
     }
 };
+
+// Tile autotuner: primitives of the current state without touching U
+// (k_c2p with store_u = 0), then substep-0 sweeps of every candidate
+// variant, timed with events.  A substep-0 sweep reads slot 0 and writes
+// only slot 1 of U and BF, which the next real substep 0 rewrites, so the
+// state the caller sees is unchanged.
+template <class FT, bool MHD> void autotune_t(kr_ctx *c) {
+    const MeshView<FT> m = c->view<FT>();
+    const unsigned nb = (unsigned)c->descs.size();
+    hipLaunchKernelGGL((k_c2p<FT, MHD>), dim3(ceil_div(c->ncell, kThreads), nb), dim3(kThreads), 0,
+                       c->stream, m, 0, 0);
+    Launcher L{c};
+    float best = 1e30f;
+    int pick = c->candidates[0];
+    for (int v : c->candidates) {
+        L.sweep<FT>(m, 0, v);                       // warm-up (code load, caches)
+        KR_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < 3; ++r) L.sweep<FT>(m, 0, v);
+        KR_HIP(hipEventRecord(c->ev1, c->stream));
+        KR_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0;
+        KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->tune_ms[v] = ms / 3;
+        if (ms < best) {
+            best = ms;
+            pick = v;
+        }
+    }
+    KR_HIP(hipGetLastError());
+    c->variant = pick;
+}
+
+void fill_tile_stats(kr_ctx *c) {
+    for (int k = 0; k < 4; ++k) c->stats.sweep_tile[k] = 0;
+    if (!c->fused) return;
+    const SweepVariant &sv = kSweepVariants[c->variant];
+    c->stats.sweep_tile[0] = sv.tx;
+    c->stats.sweep_tile[1] = sv.ty;
+    c->stats.sweep_tile[2] = sv.nt;
+    c->stats.sweep_tile[3] = sv.waves;
+    for (int v = 0; v < kNumSweepVariants && v < 8; ++v) c->stats.tune_ms[v] = c->tune_ms[v];
+}
+
+void autotune(kr_ctx *c) {
+    if (c->tuned || !c->fused) return;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (c->mixed) autotune_t<float, true>(c);
+    else autotune_t<double, true>(c);
+    c->tuned = true;
+}
 
 int fail(kr_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -594,6 +674,7 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
 int kr_step(kr_ctx *c, double *dt_taken) {
     return guarded(c, [&] {
         commit(c);
+        autotune(c);
         const auto before = read_all_ts(c);
         KR_HIP(hipEventRecord(c->ev0, c->stream));
         Launcher L{c};
@@ -604,6 +685,7 @@ int kr_step(kr_ctx *c, double *dt_taken) {
         KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         const auto after = read_all_ts(c);
         c->stats.gpu_ms = ms;
+        fill_tile_stats(c);
         c->stats.cycles = cycles_between(before, after);
         if (dt_taken)
             *dt_taken = after[0].cycles != before[0].cycles ? after[0].dt_use : 0.0;
@@ -615,6 +697,7 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
     return guarded(c, [&] {
         commit(c);
         if (n_cycles < 0) throw std::invalid_argument("n_cycles < 0");
+        autotune(c);
         if (!c->graph) {
             hipGraph_t g;
             KR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -644,6 +727,7 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
         }
         const auto after = read_all_ts(c);
         c->stats.gpu_ms = ms_tot;
+        fill_tile_stats(c);
         c->stats.cycles = cycles_between(before, after);
         if (n_done) *n_done = c->stats.cycles;
         return KR_OK;
@@ -653,6 +737,7 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
 int kr_profile(kr_ctx *c, int n_cycles) {
     return guarded(c, [&] {
         commit(c);
+        autotune(c);
         Launcher L{c, true};
         for (int i = 0; i < n_cycles; ++i) L.cycle();
         KR_HIP(hipStreamSynchronize(c->stream));

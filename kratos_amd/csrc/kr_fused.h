@@ -106,8 +106,11 @@ __device__ __forceinline__ FT edge_emf(FT ec11, FT ec10, FT ec00, FT ec01, FT ef
     return ee;
 }
 
-template <class FT, int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
+// WAVES > 1 asks the compiler for at least that many waves per SIMD (fewer
+// VGPRs, possibly spills); the tile autotuner times both forms.
+template <class FT, int TX, int TY, int NT, int WAVES>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_sweep(MeshView<FT> m, int step) {
     using S = TileShape<TX, TY>;
     static_assert(S::NCOL <= NT, "one halo column per thread");
     static_assert(S::NZI <= NT, "one z pencil per thread");
@@ -203,22 +206,7 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
         __syncthreads();
         if (L + 1 <= nz + 1) prefetch_w(L + 1);
 
-        // prefetch the conserved state of the plane being started (L-1) and
-        // the face fields of the CT of this iteration
         const int P = L - 2;          // plane finished in this iteration
-        const int Pn = L - 1;         // plane started (x/y partial)
-        double us[5] = {0, 0, 0, 0, 0}, ut[5] = {0, 0, 0, 0, 0};
-        if (zint && Pn >= 0 && Pn < nz) {
-            const size_t c = cidx(m.n, x0 + cx_, y0 + cy_, Pn);
-            const double *Us = m.U + ((size_t)b * 2 + step) * 5 * nc + c;
-            const double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + c;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) us[v] = Us[(size_t)v * nc];
-            if (step == 1) {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) ut[v] = Ut[(size_t)v * nc];
-            }
-        }
 
         // ------------------------------------------------ 2. fluxes
         if (L >= 0) {
@@ -438,13 +426,18 @@ __global__ void __launch_bounds__(NT) k_sweep(MeshView<FT> m, int step) {
                 const int x = cx_, y = cy_;
                 // x + y partial of plane q (cells), finished next iteration
                 if (q < nz) {
+                    // conserved state of plane q, loaded here so that it is not
+                    // live across the flux phase
+                    const size_t c = cidx(m.n, x0 + x, y0 + y, q);
+                    const double *Us = m.U + ((size_t)b * 2 + step) * 5 * nc + c;
+                    const double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + c;
                     double uu[5];
                     if (step == 0) {
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) uu[v] = us[v];
+                        for (int v = 0; v < 5; ++v) uu[v] = Us[(size_t)v * nc];
                     } else {
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) uu[v] = (us[v] + ut[v]) * 0.5f;
+                        for (int v = 0; v < 5; ++v) uu[v] = (Us[(size_t)v * nc] + Ut[(size_t)v * nc]) * 0.5f;
                     }
 #pragma unroll
                     for (int a = 0; a < 2; ++a) {

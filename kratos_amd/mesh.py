@@ -197,7 +197,7 @@ class Context:
     """One device context of the C-ABI over a uniform block mesh."""
 
     def __init__(self, spec: MeshSpec, phys: Physics, precision="fp64", path="auto",
-                 tile=(0, 0), members=1):
+                 tile=(0, 0, 0), members=1):
         """`members` > 1 makes an ensemble: that many independent copies of
         the mesh in one context, each its own time group (own t and dt),
         all advanced by the same launches.  upload / download / time take
@@ -219,7 +219,9 @@ class Context:
         for a in range(3):
             cfg.n_cell_block[a] = spec.n_cell_block[a]
         cfg.kernel_path = _capi.PATHS[path]
-        cfg.tile[0], cfg.tile[1] = int(tile[0]), int(tile[1])
+        tile = tuple(tile) + (0,) * (3 - len(tile))
+        for k in range(3):
+            cfg.tile[k] = int(tile[k])
         h = C.c_void_p()
         rc = lib.kr_create(C.byref(cfg), C.byref(h))
         _capi.check(rc, None, "kr_create")
@@ -306,4 +308,6 @@ class Context:
             "kernel_ms": {k: s.kernel_ms[i] for i, k in enumerate(_capi.KERNEL_KINDS)},
             "kernel_calls": {k: s.kernel_calls[i] for i, k in enumerate(_capi.KERNEL_KINDS)},
             "alg_bytes_per_cell_update": s.alg_bytes_per_cell_update,
+            "sweep_tile": tuple(s.sweep_tile),
+            "tune_ms": tuple(s.tune_ms),
         }

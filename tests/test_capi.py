@@ -56,7 +56,8 @@ int main(void) {
     F(kr_config, n_cell_block); F(kr_config, kernel_path); F(kr_config, tile);
     F(kr_block_desc, xf0); F(kr_block_desc, dx0); F(kr_block_desc, nbr);
     F(kr_stats, n_cells); F(kr_stats, kernel_ms); F(kr_stats, kernel_calls);
-    F(kr_stats, alg_bytes_per_cell_update);
+    F(kr_stats, alg_bytes_per_cell_update); F(kr_stats, sweep_tile); F(kr_stats, tune_ms);
+    F(kr_block_desc, group);
     return 0;
 }
 """

@@ -133,3 +133,68 @@ def test_ensemble_per_box_dt(gpu_lib, path):
             np.testing.assert_array_equal(x, y)
     ctx.close()
     assert len(set(cycles)) > 1, "boxes should need different cycle counts"
+
+
+_ORACLE_CACHE = {}
+
+
+def _oracle_state(N, blk, n_cycles, seed):
+    key = (N, blk, n_cycles, seed)
+    if key not in _ORACLE_CACHE:
+        spec = kratos_amd.MeshSpec(N, blk)
+        phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+        u, b, _ = problems.turbulence(N, seed=seed)
+        ref = oracle.OracleMesh(spec, phys)
+        ref.upload(u, b)
+        ref.advance(n_cycles)
+        _ORACLE_CACHE[key] = (u, b, ref.download())
+    return _ORACLE_CACHE[key]
+
+
+@pytest.mark.parametrize("tile", [(32, 8, 1), (32, 8, 4), (16, 16, 1), (16, 16, 4), (16, 8, 1),
+                                  (8, 8, 1)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_tile_variants(gpu_lib, tile, precision):
+    """Every fused-kernel variant the autotuner may pick, forced one at a
+    time: FP64 bit-exact with the oracle, mixed within the L1 bar."""
+    if precision == "fp64" and tile[2] != 1:
+        pytest.skip("waves-per-SIMD variants exist for the FP32 flux form only")
+    N, blk = (64, 64, 64), (32, 32, 32)
+    u, b, (ou, ob) = _oracle_state(N, blk, 3, 4)
+    spec = kratos_amd.MeshSpec(N, blk)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    print(f"\n[case] fused tile {tile} {precision}", flush=True)
+    ctx = kratos_amd.Context(spec, phys, precision, path="fused", tile=tile)
+    ctx.upload(u, b)
+    assert ctx.advance(3) == 3
+    st = ctx.stats()
+    assert st["sweep_tile"][0:2] == tile[0:2] and st["sweep_tile"][3] == tile[2]
+    gu, gb = ctx.download()
+    ctx.close()
+    if precision == "fp64":
+        np.testing.assert_array_equal(gu, ou)
+        for x, y in zip(gb, ob):
+            np.testing.assert_array_equal(x, y)
+    else:
+        assert rel_l1(gu[0], ou[0]) <= 1e-5
+        cat = lambda bs: np.concatenate([x.ravel() for x in bs])
+        assert rel_l1(cat(gb), cat(ob)) <= 1e-5
+
+
+def test_autotuner_keeps_state(gpu_lib):
+    """The autotuner's timing sweeps leave the state untouched: an
+    autotuned run equals a forced-variant run bit for bit (FP64)."""
+    N, blk = (64, 64, 64), (32, 32, 32)
+    u, b, (ou, ob) = _oracle_state(N, blk, 3, 4)
+    spec = kratos_amd.MeshSpec(N, blk)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    ctx = kratos_amd.Context(spec, phys, "fp64", path="fused")
+    ctx.upload(u, b)
+    assert ctx.advance(3) == 3
+    st = ctx.stats()
+    assert sum(1 for t in st["tune_ms"] if t > 0) >= 2, st["tune_ms"]
+    gu, gb = ctx.download()
+    ctx.close()
+    np.testing.assert_array_equal(gu, ou)
+    for x, y in zip(gb, ob):
+        np.testing.assert_array_equal(x, y)
