@@ -212,7 +212,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=8,
+                    help="untimed cycles; the first ones also run the tile autotuner")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--block", type=int, default=32)
     ap.add_argument("--forms", default="fp64,mixed")

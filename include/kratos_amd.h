@@ -48,8 +48,9 @@ typedef struct kr_config {
                               kernel when the block shape allows it         */
     int    tile[3];        /* fused-kernel column tile (x, y) and minimum
                               waves per SIMD; 0 entries are left to the
-                              autotuner, which times every fitting variant
-                              at the first step                             */
+                              autotuner, which runs the first cycles with
+                              one fitting variant each and keeps the
+                              fastest (every variant gives the same state) */
 } kr_config;
 
 enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };
@@ -78,8 +79,8 @@ typedef struct kr_stats {
     double alg_bytes_per_cell_update;  /* algorithmic HBM bytes            */
     int    sweep_tile[4];  /* fused kernel in use: tile x, y, threads, waves
                               per SIMD asked (zeros: staged path)           */
-    float  tune_ms[8];     /* autotuner: mean substep-0 sweep time per
-                              variant (0: not timed)                        */
+    float  tune_ms[8];     /* autotuner: mean stage-kernel time per variant
+                              over its tuning cycle (0: not timed)          */
 } kr_stats;
 
 enum kr_kernel_kind {

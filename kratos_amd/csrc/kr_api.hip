@@ -121,7 +121,8 @@ struct kr_ctx {
     bool mhd = false, mixed = false;
     bool fused = false;
     int variant = -1;               // kSweepVariants index of the fused kernel
-    std::vector<int> candidates;    // variants left for the autotuner
+    std::vector<int> candidates;    // variants the autotuner times
+    int tune_next = 0;              // next candidate to time
     bool tuned = false;
     int n[3] = {0, 0, 0};
     int ndim = 0, nv = 5;
@@ -314,6 +315,7 @@ void commit(kr_ctx *c) {
             throw std::invalid_argument("no fused tile fits the block shape");
         c->fused = !c->candidates.empty();
         if (c->fused) c->variant = c->candidates[0];
+        c->tune_next = 0;
         c->tuned = c->candidates.size() <= 1;
     }
     c->stats.n_blocks = (int)nb;
@@ -326,11 +328,12 @@ void commit(kr_ctx *c) {
 // -------------------------------------------------------------- launches
 struct Launcher {
     kr_ctx *c;
-    bool prof = false;
+    bool prof = false;   // events around every launch
+    bool tune = false;   // events around the fused stage-kernel launches only
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
 
     template <class F> void run(int kind, F &&f) {
-        if (!prof) {
+        if (!prof && !(tune && kind == KR_K_UPDATE && c->fused)) {
             f();
             return;
         }
@@ -430,37 +433,6 @@ struct Launcher {
     }
 };
 
-// Tile autotuner: primitives of the current state without touching U
-// (k_c2p with store_u = 0), then substep-0 sweeps of every candidate
-// variant, timed with events.  A substep-0 sweep reads slot 0 and writes
-// only slot 1 of U and BF, which the next real substep 0 rewrites, so the
-// state the caller sees is unchanged.
-template <class FT, bool MHD> void autotune_t(kr_ctx *c) {
-    const MeshView<FT> m = c->view<FT>();
-    const unsigned nb = (unsigned)c->descs.size();
-    hipLaunchKernelGGL((k_c2p<FT, MHD>), dim3(ceil_div(c->ncell, kThreads), nb), dim3(kThreads), 0,
-                       c->stream, m, 0, 0);
-    Launcher L{c};
-    float best = 1e30f;
-    int pick = c->candidates[0];
-    for (int v : c->candidates) {
-        L.sweep<FT>(m, 0, v);                       // warm-up (code load, caches)
-        KR_HIP(hipEventRecord(c->ev0, c->stream));
-        for (int r = 0; r < 3; ++r) L.sweep<FT>(m, 0, v);
-        KR_HIP(hipEventRecord(c->ev1, c->stream));
-        KR_HIP(hipEventSynchronize(c->ev1));
-        float ms = 0;
-        KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        c->tune_ms[v] = ms / 3;
-        if (ms < best) {
-            best = ms;
-            pick = v;
-        }
-    }
-    KR_HIP(hipGetLastError());
-    c->variant = pick;
-}
-
 void fill_tile_stats(kr_ctx *c) {
     for (int k = 0; k < 4; ++k) c->stats.sweep_tile[k] = 0;
     if (!c->fused) return;
@@ -472,15 +444,35 @@ void fill_tile_stats(kr_ctx *c) {
     for (int v = 0; v < kNumSweepVariants && v < 8; ++v) c->stats.tune_ms[v] = c->tune_ms[v];
 }
 
-void autotune(kr_ctx *c) {
-    if (c->tuned || !c->fused) return;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
+// Tile autotuner on real cycles: while candidates are left, each cycle runs
+// the next candidate with events around its two stage-kernel launches; when
+// all are timed the fastest is kept.  Every variant performs the same
+// arithmetic per face and per cell in the same order, so tuning cycles leave
+// exactly the state any fixed choice would.
+bool tuning(const kr_ctx *c) { return c->fused && !c->tuned; }
+
+void tune_cycle(kr_ctx *c) {
+    const int v = c->candidates[c->tune_next];
+    c->variant = v;
+    Launcher L{c, false, true};
+    L.cycle();
+    KR_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    for (auto &e : L.evs) {
+        float t = 0;
+        KR_HIP(hipEventElapsedTime(&t, e.second.first, e.second.second));
+        ms += t;
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
     }
-    if (c->mixed) autotune_t<float, true>(c);
-    else autotune_t<double, true>(c);
-    c->tuned = true;
+    c->tune_ms[v] = L.evs.empty() ? 0.f : ms / (float)L.evs.size();
+    if (++c->tune_next == (int)c->candidates.size()) {
+        int pick = c->candidates[0];
+        for (int w : c->candidates)
+            if (c->tune_ms[w] < c->tune_ms[pick]) pick = w;
+        c->variant = pick;
+        c->tuned = true;
+    }
 }
 
 int fail(kr_ctx *c, int code, const std::string &msg) {
@@ -674,11 +666,14 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
 int kr_step(kr_ctx *c, double *dt_taken) {
     return guarded(c, [&] {
         commit(c);
-        autotune(c);
         const auto before = read_all_ts(c);
         KR_HIP(hipEventRecord(c->ev0, c->stream));
-        Launcher L{c};
-        L.cycle();
+        if (tuning(c)) {
+            tune_cycle(c);
+        } else {
+            Launcher L{c};
+            L.cycle();
+        }
         KR_HIP(hipEventRecord(c->ev1, c->stream));
         KR_HIP(hipStreamSynchronize(c->stream));
         float ms = 0;
@@ -697,8 +692,23 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
     return guarded(c, [&] {
         commit(c);
         if (n_cycles < 0) throw std::invalid_argument("n_cycles < 0");
-        autotune(c);
-        if (!c->graph) {
+        const auto before = read_all_ts(c);
+        double ms_tot = 0;
+        int left = n_cycles;
+        bool done = all_done(before);
+        // tuning cycles first (eager launches), then the captured graph
+        while (left > 0 && !done && tuning(c)) {
+            KR_HIP(hipEventRecord(c->ev0, c->stream));
+            tune_cycle(c);
+            KR_HIP(hipEventRecord(c->ev1, c->stream));
+            KR_HIP(hipEventSynchronize(c->ev1));
+            float ms = 0;
+            KR_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            ms_tot += ms;
+            --left;
+            done = all_done(read_all_ts(c));
+        }
+        if (left > 0 && !done && !c->graph) {
             hipGraph_t g;
             KR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             Launcher L{c};
@@ -709,10 +719,6 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
         }
         // replays in chunks; between chunks the host stops early once every
         // group has reached t_lim (later replays would skip all work)
-        const auto before = read_all_ts(c);
-        double ms_tot = 0;
-        int left = n_cycles;
-        bool done = all_done(before);
         while (left > 0 && !done) {
             const int chunk = left < kAdvanceChunk ? left : kAdvanceChunk;
             KR_HIP(hipEventRecord(c->ev0, c->stream));
@@ -737,7 +743,6 @@ int kr_advance(kr_ctx *c, int n_cycles, int *n_done) {
 int kr_profile(kr_ctx *c, int n_cycles) {
     return guarded(c, [&] {
         commit(c);
-        autotune(c);
         Launcher L{c, true};
         for (int i = 0; i < n_cycles; ++i) L.cycle();
         KR_HIP(hipStreamSynchronize(c->stream));

@@ -31,6 +31,7 @@ struct TileShape {
     static constexpr int NF = NFX + NFY;
     static constexpr int RX = TX + 2, NZG = (TX + 2) * (TY + 2);      // z-face ring grid
     static constexpr int NZI = TX * TY + 2 * (TX + 2) + 2 * TY;       // ring items
+    static constexpr int NZIP = (NZI + 63) / 64 * 64;                 // ... padded to waves
     static constexpr int NEZ = (TX + 1) * (TY + 1), NEX = TX * (TY + 1), NEY = (TX + 1) * TY;
     __device__ static int col(int x, int y) { return (y + 2) * HX + (x + 2); }
     __device__ static int zg(int x, int y) { return (y + 1) * RX + (x + 1); }
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))
 k_sweep(MeshView<FT> m, int step) {
     using S = TileShape<TX, TY>;
     static_assert(S::NCOL <= NT, "one halo column per thread");
-    static_assert(S::NZI <= NT, "one z pencil per thread");
+    static_assert(S::NZIP <= NT, "one z pencil per thread, in-plane items from a wave boundary");
     static_assert(TX * TY + TX + TY <= NT, "one owned cell / edge face per thread");
     if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     const int tid = threadIdx.x;
@@ -287,8 +288,11 @@ k_sweep(MeshView<FT> m, int step) {
 #pragma unroll
                 for (int v = 0; v < 8; ++v) qL[v] = qLn[v];
             }
-            // in-plane faces of plane p: items beyond the z pencils
-            for (int it = tid - S::NZI + (tid < S::NZI ? NT : 0); it < S::NF; it += NT) {
+            // in-plane faces of plane p.  The z pencils occupy threads
+            // 0 .. NZI-1; the in-plane items start at the next wave boundary
+            // NZIP, so that every wave runs the Riemann solver the same
+            // number of times (thread t starts at item (t - NZIP) mod NT).
+            for (int it = tid - S::NZIP + (tid < S::NZIP ? NT : 0); it < S::NF; it += NT) {
                 if (it < 0) continue;
                 const bool xf = it < S::NFX;
                 int fx, fy;

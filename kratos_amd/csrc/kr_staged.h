@@ -144,10 +144,8 @@ __device__ __forceinline__ double block_min(double v) {
 // hydro/eos/adiabatic.h:211-259 and mhd/eos/adiabatic.h:127-235: primitives
 // with floors written back to the conserved slot, cell-centred B from the
 // face average, and at substep 0 the per-cell CFL dt reduced per workgroup.
-// store_u = 0 leaves U untouched (primitives only; the tile autotuner uses
-// it so that timing runs do not modify the state).
 template <class FT, bool MHD>
-__global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step, int store_u = 1) {
+__global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
     if (m.ts[m.geo[blockIdx.y].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
     const int b = blockIdx.y;
@@ -186,10 +184,8 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step, int 
             cons[ENE] -= e_m;
             c2p(cons, prim, e, hr, he);
             cons[ENE] += e_m;
-            if (store_u) {
-                U[(size_t)ENE * m.ncell] = cons[ENE];
-                if (hr) U[0] = cons[RHO];
-            }
+            U[(size_t)ENE * m.ncell] = cons[ENE];
+            if (hr) U[0] = cons[RHO];
 #pragma unroll
             for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
 #pragma unroll
@@ -197,8 +193,8 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step, int 
             if (step == 0) dtc = dt_mhd(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl);
         } else {
             const double cs2 = c2p(cons, prim, e, hr, he);
-            if (store_u && hr) U[0] = cons[RHO];
-            if (store_u && he) U[(size_t)ENE * m.ncell] = cons[ENE];
+            if (hr) U[0] = cons[RHO];
+            if (he) U[(size_t)ENE * m.ncell] = cons[ENE];
 #pragma unroll
             for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
             if (step == 0) dtc = dt_hydro(prim, cs2, g.dx, m.ndim, m.cfl);

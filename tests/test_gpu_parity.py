@@ -182,17 +182,21 @@ def test_fused_tile_variants(gpu_lib, tile, precision):
 
 
 def test_autotuner_keeps_state(gpu_lib):
-    """The autotuner's timing sweeps leave the state untouched: an
-    autotuned run equals a forced-variant run bit for bit (FP64)."""
+    """The autotuner runs the first cycles with a different fused-kernel
+    variant each; the state must still equal the oracle's bit for bit (FP64),
+    before and after the choice is made."""
     N, blk = (64, 64, 64), (32, 32, 32)
-    u, b, (ou, ob) = _oracle_state(N, blk, 3, 4)
+    u, b, (ou, ob) = _oracle_state(N, blk, 6, 4)
     spec = kratos_amd.MeshSpec(N, blk)
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     ctx = kratos_amd.Context(spec, phys, "fp64", path="fused")
     ctx.upload(u, b)
+    assert ctx.advance(2) == 2
+    assert ctx.step() > 0
     assert ctx.advance(3) == 3
     st = ctx.stats()
-    assert sum(1 for t in st["tune_ms"] if t > 0) >= 2, st["tune_ms"]
+    timed = [t for t in st["tune_ms"] if t > 0]
+    assert len(timed) == 4, st["tune_ms"]          # 32x8, 16x16, 16x8, 8x8
     gu, gb = ctx.download()
     ctx.close()
     np.testing.assert_array_equal(gu, ou)
