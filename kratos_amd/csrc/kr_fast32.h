@@ -7,7 +7,9 @@
 // branch-free: every face computes the L/R star states and the double-star
 // state of its side, and the region (supersonic, star, double star) only
 // selects the weights.  Wave speeds and intermediate states are those of
-// hlld<FT> (mhd/riemann/hlld.h:81-298, Miyoshi & Kusano 2005).
+// hlld<FT> (mhd/riemann/hlld.h:81-298, Miyoshi & Kusano 2005).  The library
+// is built with -ffp-contract=off for the FP64 form's bit parity; these FP32
+// functions opt back in to fused multiply-add contraction.
 // Included by kr_physics.h after its variable indices (RHO, ENE, MOM, BCC).
 #pragma once
 
@@ -20,6 +22,7 @@ __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x)
 // Harmonic (van Leer) PLM increment of one cell: dl*dr/(dl+dr) where the
 // differences agree in sign, else 0 (hydro/reconstruct/plm.h:116-153).
 __device__ __forceinline__ float plm_inc_fast(float qm, float q0, float qp) {
+#pragma clang fp contract(fast)
     const float dl = q0 - qm, dr = qp - q0;
     const float pr = dl * dr;
     return pr > 0.f ? pr * frcp(dl + dr) : 0.f;
@@ -29,6 +32,7 @@ __device__ __forceinline__ float plm_inc_fast(float qm, float q0, float qp) {
 // ll, l, r, rr (the per-face form of plm<float>).
 __device__ __forceinline__ void plm_fast(float q_ll, float q_l, float q_r, float q_rr, float &ql,
                                          float &qr) {
+#pragma clang fp contract(fast)
     ql = q_l + plm_inc_fast(q_ll, q_l, q_r);
     qr = q_r - plm_inc_fast(q_l, q_r, q_rr);
 }
@@ -40,6 +44,7 @@ struct Side32 {
 };
 
 __device__ __forceinline__ Side32 side32(const float *q, float bx2, float gam, float gm1i) {
+#pragma clang fp contract(fast)
     Side32 s;
     s.r = q[RHO];
     s.p = q[ENE];
@@ -59,15 +64,16 @@ __device__ __forceinline__ Side32 side32(const float *q, float bx2, float gam, f
     return s;
 }
 
-// Star state of one side: returns (rho*, my*, mz*, by*, bz*, E*) and u.b*.
+// Transverse momentum and field of one side's star state (hlld.h:160-190):
+// the Alfven-degenerate case (|denom| < f_tiny p*_T) keeps v_t and B_t.
 struct Star32 {
-    float r, my, mz, by, bz, e, udb;
+    float my, mz, by, bz;
 };
 
 __device__ __forceinline__ Star32 star32(const Side32 &s, float bx, float bx2, float sd,
-                                         float smd, float rs, float sm, float pts, float tiny) {
+                                         float smd, float rs, float pts, float tiny) {
+#pragma clang fp contract(fast)
     Star32 o;
-    o.r = rs;
     const float rsd = s.r * sd;
     const float denom = rsd * smd - bx2;
     const bool deg = fabsf(denom) < tiny * pts;
@@ -78,17 +84,16 @@ __device__ __forceinline__ Star32 star32(const Side32 &s, float bx, float bx2, f
     o.mz = deg ? rs * s.w : rs * (s.w - s.bz * tv);
     o.by = deg ? s.by : s.by * tb;
     o.bz = deg ? s.bz : s.bz * tb;
-    const float irs = frcp(rs);
-    o.udb = (rs * sm * bx + o.my * o.by + o.mz * o.bz) * irs;
-    const float wdb = s.u * bx + s.v * s.by + s.w * s.bz;
-    o.e = (bx * (wdb - o.udb) - s.pt * s.u + sd * s.e + pts * sm) * frcp(smd);
     return o;
 }
 
 // FP32 HLLD flux in the rotated frame (wl / wr: rho, p, v_n, v_t1, v_t2,
 // B_n, B_t1, B_t2).  flx slots as in hlld<FT>: mass, energy, momentum,
-// B_n (0), B_t1, B_t2.
+// B_n (0), B_t1, B_t2.  The energy of the star state and the physical flux
+// are formed for the upwind side X of the contact only (the flux is
+// F_X + a (U*_X - U_X) + c (U**_X - U*_X)).
 __device__ __forceinline__ void hlld_fast(float *flx, const float *wl, const float *wr, float gam) {
+#pragma clang fp contract(fast)
     const float gm1i = frcp(gam - 1.f);
     const float bx = wl[BCC], bx2 = bx * bx;
     const Side32 L = side32(wl, bx2, gam, gm1i);
@@ -102,63 +107,65 @@ __device__ __forceinline__ void hlld_fast(float *flx, const float *wl, const flo
     const float rsl = rsdl * frcp(sml), rsr = rsdr * frcp(smr);
     const float isql = frsq(rsl), isqr = frsq(rsr);
     const float sql = rsl * isql, sqr = rsr * isqr;
+    const float irsl = isql * isql, irsr = isqr * isqr;       // 1 / rho*
     const float abx = fabsf(bx);
     const float ssl = sm - abx * isql, ssr = sm + abx * isqr;
     const float tiny = 1.0e-4f;
     const float pts = L.pt + rsdl * (sdl - sml);
-    const Star32 SL = star32(L, bx, bx2, sdl, sml, rsl, sm, pts, tiny);
-    const Star32 SR = star32(R, bx, bx2, sdr, smr, rsr, sm, pts, tiny);
+    const Star32 SL = star32(L, bx, bx2, sdl, sml, rsl, pts, tiny);
+    const Star32 SR = star32(R, bx, bx2, sdr, smr, rsr, pts, tiny);
 
-    const bool left = sm >= 0.f;
-    // double-star state of the chosen side (equal to the star state when
-    // B_n^2 is below tiny * p*_T)
+    // double-star transverse velocity and field (hlld.h:246-276)
     const float sgn = bx > 0.f ? 1.f : -1.f;
     const float inv = frcp(sql + sqr);
     const float vy = inv * (SL.my * isql + SR.my * isqr + sgn * (SR.by - SL.by));
     const float vz = inv * (SL.mz * isql + SR.mz * isqr + sgn * (SR.bz - SL.bz));
-    const float byss = inv * (sql * SR.by + sqr * SL.by +
-                              sgn * sql * sqr * (SR.my * frcp(rsr) - SL.my * frcp(rsl)));
-    const float bzss = inv * (sql * SR.bz + sqr * SL.bz +
-                              sgn * sql * sqr * (SR.mz * frcp(rsr) - SL.mz * frcp(rsl)));
+    const float sgq = sgn * sql * sqr;
+    const float byss = inv * (sql * SR.by + sqr * SL.by + sgq * (SR.my * irsr - SL.my * irsl));
+    const float bzss = inv * (sql * SR.bz + sqr * SL.bz + sgq * (SR.mz * irsr - SL.mz * irsl));
     const float tmp = sm * bx + (vy * byss + vz * bzss);
     const bool dstar = bx2 > pts * tiny;
 
-    const Side32 &X = left ? L : R;
-    const Star32 &S = left ? SL : SR;
-    const float sqx = left ? sql : sqr;
-    const float rsx = left ? rsl : rsr;
+    // upwind side of the contact
+    const bool left = sm >= 0.f;
+    const float xr = left ? L.r : R.r, xu = left ? L.u : R.u, xv = left ? L.v : R.v;
+    const float xw = left ? L.w : R.w, xby = left ? L.by : R.by, xbz = left ? L.bz : R.bz;
+    const float xe = left ? L.e : R.e, xpt = left ? L.pt : R.pt;
+    const float sdx = left ? sdl : sdr, smdx = left ? sml : smr;
+    const float rsx = left ? rsl : rsr, irsx = left ? irsl : irsr, sqx = left ? sql : sqr;
+    const float mysx = left ? SL.my : SR.my, mzsx = left ? SL.mz : SR.mz;
+    const float bysx = left ? SL.by : SR.by, bzsx = left ? SL.bz : SR.bz;
     const float s1 = left ? sl : sr;
     const float s2 = left ? ssl : ssr;
-    const float ess = left ? S.e - sqx * sgn * (S.udb - tmp) : S.e + sqx * sgn * (S.udb - tmp);
+    // star energy of side X (hlld.h:192-200)
+    const float vbx = xu * bx + xv * xby + xw * xbz;
+    const float udb = (rsx * sm * bx + mysx * bysx + mzsx * bzsx) * irsx;
+    const float esx = (bx * (vbx - udb) - xpt * xu + sdx * xe + pts * sm) * frcp(smdx);
+    const float sx = left ? -sgn : sgn;
+    const float ess = esx + sqx * sx * (udb - tmp);
     // double-star conserved (falls back to the star state)
-    const float mys = dstar ? rsx * vy : S.my;
-    const float mzs = dstar ? rsx * vz : S.mz;
-    const float bys = dstar ? byss : S.by;
-    const float bzs = dstar ? bzss : S.bz;
-    const float es = dstar ? ess : S.e;
-    // weights: F = F_X + a (U*_X - U_X) + c (U**_X - U*_X)
+    const float mys = dstar ? rsx * vy : mysx;
+    const float mzs = dstar ? rsx * vz : mzsx;
+    const float bys = dstar ? byss : bysx;
+    const float bzs = dstar ? bzss : bzsx;
+    const float es = dstar ? ess : esx;
     const bool super_ = sl >= 0.f || sr <= 0.f;
-    float a = left ? (s1 < 0.f ? s1 : 0.f) : (s1 > 0.f ? s1 : 0.f);
-    float c = left ? (s2 < 0.f ? s2 : 0.f) : (s2 > 0.f ? s2 : 0.f);
+    const float a = left ? (s1 < 0.f ? s1 : 0.f) : (s1 > 0.f ? s1 : 0.f);
+    const float c = left ? (s2 < 0.f ? s2 : 0.f) : (s2 > 0.f ? s2 : 0.f);
     // physical flux of side X
-    const float vbx = X.u * bx + X.v * X.by + X.w * X.bz;
-    const float mx = X.r * X.u;
+    const float mx = xr * xu;
     float f[7];
     f[0] = mx;
-    f[1] = X.u * (X.e + X.pt) - bx * vbx;
-    f[2] = mx * X.u + X.pt - bx2;
-    f[3] = mx * X.v - bx * X.by;
-    f[4] = mx * X.w - bx * X.bz;
-    f[5] = X.by * X.u - bx * X.v;
-    f[6] = X.bz * X.u - bx * X.w;
+    f[1] = xu * (xe + xpt) - bx * vbx;
+    f[2] = mx * xu + xpt - bx2;
+    f[3] = mx * xv - bx * xby;
+    f[4] = mx * xw - bx * xbz;
+    f[5] = xby * xu - bx * xv;
+    f[6] = xbz * xu - bx * xw;
     // U_X, U*_X, U**_X
-    const float ux[7] = {X.r, X.e, mx, X.r * X.v, X.r * X.w, X.by, X.bz};
-    const float usx[7] = {rsx, S.e, rsx * sm, S.my, S.mz, S.by, S.bz};
+    const float ux[7] = {xr, xe, mx, xr * xv, xr * xw, xby, xbz};
+    const float usx[7] = {rsx, esx, rsx * sm, mysx, mzsx, bysx, bzsx};
     const float ussx[7] = {rsx, es, rsx * sm, mys, mzs, bys, bzs};
-    if (super_) {
-        a = 0.f;
-        c = 0.f;
-    }
     float g[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) {

@@ -83,13 +83,49 @@ __device__ __forceinline__ FT plm_inc(FT qm, FT q0, FT qp, bool &flat) {
     return coef * (dql + dqc);
 }
 
+// a * b - c * d in FP32 with multiply-add contraction (mixed form)
+__device__ __forceinline__ float cross_f32(float a, float b, float c, float d) {
+#pragma clang fp contract(fast)
+    return a * b - c * d;
+}
+
 // Edge EMF from the four surrounding cell E values and the two pairs of
 // face EMFs / upwind mass fluxes (mhd/integrator/integrator.h:244-317).
 // ec[s2][s1] naming of the reference: ec11 = E(i), ec10 = E(i - e1),
 // ec00 = E(i - e1 - e2), ec01 = E(i - e2).
 template <class FT>
 __device__ __forceinline__ FT edge_emf(FT ec11, FT ec10, FT ec00, FT ec01, FT efm1, FT efp1,
-                                       FT rp1, FT rm1, FT efm2, FT efp2, FT rp2, FT rm2) {
+                                       FT rp1, FT rm1, FT efm2, FT efp2, FT rp2, FT rm2);
+
+// FP32 (mixed form): the same expression with multiply-add contraction.
+template <>
+__device__ __forceinline__ float edge_emf<float>(float ec11, float ec10, float ec00, float ec01,
+                                                 float efm1, float efp1, float rp1, float rm1,
+                                                 float efm2, float efp2, float rp2, float rm2) {
+#pragma clang fp contract(fast)
+    const float w_e = 0.25f;
+    float ee = 0;
+    ee += w_e * (efm1 + efp1);
+    {
+        const float de_p = (rp1 > 0 ? (ec01 - efm1) : (ec11 - efp1));
+        const float de_m = (rm1 > 0 ? -(ec00 - efm1) : -(ec10 - efp1));
+        ee += w_e * (de_m - de_p);
+    }
+    ee += w_e * (efm2 + efp2);
+    {
+        const float de_p = (rp2 > 0 ? (ec10 - efm2) : (ec11 - efp2));
+        const float de_m = (rm2 > 0 ? -(ec00 - efm2) : -(ec01 - efp2));
+        ee += w_e * (de_m - de_p);
+    }
+    return ee;
+}
+
+template <>
+__device__ __forceinline__ double edge_emf<double>(double ec11, double ec10, double ec00,
+                                                   double ec01, double efm1, double efp1,
+                                                   double rp1, double rm1, double efm2,
+                                                   double efp2, double rp2, double rm2) {
+    using FT = double;
     const FT w_e = (FT)0.25f;
     FT ee = 0;
     ee += w_e * (efm1 + efp1);
@@ -362,8 +398,12 @@ k_sweep(MeshView<FT> m, int step) {
             auto ec = [&](int s, int x, int y, int c) -> FT {
                 const int cc = S::col(x, y);
                 const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-                return sW[s][BCC + c1][cc] * sW[s][MOM + c2][cc] -
-                       sW[s][BCC + c2][cc] * sW[s][MOM + c1][cc];
+                const FT b1 = sW[s][BCC + c1][cc], v2 = sW[s][MOM + c2][cc];
+                const FT b2 = sW[s][BCC + c2][cc], v1 = sW[s][MOM + c1][cc];
+                if constexpr (std::is_same_v<FT, float>)
+                    return cross_f32(b1, v2, b2, v1);
+                else
+                    return b1 * v2 - b2 * v1;
             };
             for (int it = tid; it < S::NEZ + S::NEX + S::NEY; it += NT) {
                 if (it < S::NEZ) {
