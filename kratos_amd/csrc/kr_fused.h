@@ -164,7 +164,13 @@ k_sweep(MeshView<FT> m, int step) {
     const FT gam = (FT)m.gamma;
     const size_t nc = m.ncell;
 
-    __shared__ FT sW[3][8][S::NCOL];
+    // W ring rows are padded to whole 64-lane dword chunks: the planes are
+    // brought in by global_load_lds (lane i of a chunk writes dword i), so
+    // every chunk a wave writes lies inside its row
+    constexpr int DPC = (int)sizeof(FT) / 4;                       // dwords per column
+    constexpr int NCHUNK = (S::NCOL * DPC + 63) / 64;              // chunks per row
+    constexpr int NCOLP = NCHUNK * 64 / DPC;                       // padded columns
+    __shared__ FT sW[3][8][NCOLP];
     __shared__ FT sFe[2][3][S::NF];     // rho, 2 EMF slots of x / y faces
     __shared__ FT sFh[5][S::NF];        // hydro fluxes of the current plane
     __shared__ FT sFz[3][S::NZG];       // rho, E_x, E_y of the current z face
@@ -195,14 +201,25 @@ k_sweep(MeshView<FT> m, int step) {
         return (FT)load_bf(m, b, step, c, p);
     };
 
-    FT wpre[8];
-    const bool wloader = tid < S::NCOL;
-    const int wcx = tid % S::HX - 2, wcy = tid / S::HX - 2;
-    auto prefetch_w = [&](int z) {
-        if (!wloader) return;
-        const FT *s = wsrc(wcx, wcy, z);
+    // Asynchronous load of plane z of W into ring slot `slot` (no staging
+    // registers).  Wave w fills chunks w, w + NT/64, ... of every variable
+    // row; lane l of chunk k carries dword k*64 + l of the row, i.e. dword
+    // `half` of column col.  Padding lanes (col >= NCOL) read column 0 and
+    // write the row's padding, which nothing reads.
+    auto issue_w = [&](int z, int slot) {
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int k = wave; k < NCHUNK; k += NT / 64) {
+            const int d = k * 64 + lane;
+            const int col = d / DPC;
+            const int half = d - col * DPC;
+            const int cc = col < S::NCOL ? col : 0;
+            const FT *src = wsrc(cc % S::HX - 2, cc / S::HX - 2, z);
 #pragma unroll
-        for (int v = 0; v < 8; ++v) wpre[v] = s[(size_t)v * nc];
+            for (int v = 0; v < 8; ++v) {
+                const char *gp = (const char *)(src + (size_t)v * nc) + 4 * half;
+                __builtin_amdgcn_global_load_lds(gp, &sW[slot][v][0] + k * (64 / DPC), 4, 0, 0);
+            }
+        }
     };
 
     // ---- per-thread persistent state
@@ -232,16 +249,11 @@ k_sweep(MeshView<FT> m, int step) {
 
     const double dt_dv = dt / g.vol;
 
-    prefetch_w(-2);
+    issue_w(-2, ring3(-2));
     for (int L = -2; L <= nz + 1; ++L) {
-        // ------------------------------------------------ 1. ring store
-        if (wloader) {
-            const int s = ring3(L);
-#pragma unroll
-            for (int v = 0; v < 8; ++v) sW[s][v][tid] = wpre[v];
-        }
-        __syncthreads();
-        if (L + 1 <= nz + 1) prefetch_w(L + 1);
+        // ------------------------------------------------ 1. plane L arrives
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's loads into the ring
+        __syncthreads();                 // ... and every other wave's
 
         const int P = L - 2;          // plane finished in this iteration
 
@@ -443,6 +455,10 @@ k_sweep(MeshView<FT> m, int step) {
             }
         }
         __syncthreads();
+
+        // plane L+1 into the slot of plane L-2, which phases 2 and 3 of this
+        // iteration were the last to read; it lands while the update runs
+        if (L + 1 <= nz + 1) issue_w(L + 1, ring3(L + 1));
 
         // ------------------------------------------------ 4. update
         if (L >= 1) {
