@@ -83,9 +83,10 @@ def sync_all(ctx):
     ctx.lib.kr_sync(ctx.h)
 
 
-def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="auto"):
+def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="auto",
+             tile=(0, 0, 0)):
     import kratos_amd
-    ctx = kratos_amd.Context(spec, phys, precision, path=path)
+    ctx = kratos_amd.Context(spec, phys, precision, path=path, tile=tile)
     ctx.upload(u, b)
     if warmup:
         ctx.advance(warmup)
@@ -134,7 +135,7 @@ def ensemble_leg(n_box, n, t_lim, path):
         ctx = kratos_amd.Context(spec, phys, form, path=path, members=n_box)
         for m, (u, b) in enumerate(ics):
             ctx.upload(u, b, member=m)
-        ctx.advance(1)                      # warm-up cycle (builds the graph)
+        ctx.advance(8)                      # untimed: autotuner cycles and graph capture
         sync_all(ctx)
         c0 = [ctx.time(member=m)[2] for m in range(n_box)]
         t0 = time.perf_counter()
@@ -227,6 +228,8 @@ def main():
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-n", type=int, default=64)
     ap.add_argument("--t-ens", type=float, default=0.05)
+    ap.add_argument("--tile", default="",
+                    help="force the fused-kernel variant as TX,TY,WAVES (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB fp32 copy kernel (calibrates the HBM "
                          "byte counters of rocprofv3 --pmc)")
@@ -252,8 +255,9 @@ def main():
     res = {}
     fused = None
     for form in forms:
+        tile = tuple(int(x) for x in args.tile.split(",")) if args.tile else (0, 0, 0)
         ctx, r = run_form(spec, phys, u, b, form, args.steps, args.warmup, args.profile_cycles,
-                          args.path)
+                          args.path, tile if form == "mixed" or tile[2] in (0, 1) else (0, 0, 0))
         r["rate"] = r["n_cells"] * r["cycles"] / r["wall"]
         r["divb"] = ctx.divb_max()
         r["t"] = ctx.time()[0]
