@@ -42,6 +42,10 @@ public:
         if (!enrolled_) enroll_default();
         mesh.init_dynamics(model(), PRECISION == 1 ? KR_MIXED : KR_FP64, p_eos->gamma,
                            p_eos->cfl, p_eos->rho_floor, p_eos->tmp_floor);
+        if (mesh.is_restart()) {
+            mesh.load_state();
+            return;
+        }
         for (size_t b = 0; b < mesh.n_blocks(); ++b) {
             auto &d = mesh.dual(b);
             init_cond(d);

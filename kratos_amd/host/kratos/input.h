@@ -6,8 +6,9 @@
 // get<T>(sec, key) throws std::runtime_error when the key is missing,
 // get<T>(sec, key, default) falls back, operator()(sec, key, t) leaves t
 // untouched and returns false.  3-vectors and sequences read their items
-// separated by blanks.  Binary (restart) parameter files are not read by
-// this build.
+// separated by blanks.  A binary file (first byte not printable) is a
+// snapshot: its stored parameters are read and [file] binary names it, the
+// restart rule of input.cpp:107-160.
 #pragma once
 
 #include <fstream>
@@ -21,6 +22,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "binary_io.h"
 #include "types.h"
 
 namespace comm {
@@ -50,8 +52,18 @@ public:
             return false;
         }
         const int c0 = in.peek();
-        if (!(c0 == EOF || (c0 >= 32 && c0 <= 126)))
-            throw std::runtime_error("input: binary parameter files are not supported: " + file_);
+        if (!(c0 == EOF || (c0 >= 32 && c0 <= 126))) {
+            std::cout << "Reading input parameters from binary: " << file_ << std::endl;
+            binary_io::base_t bio;
+            bio.load(file_);
+            for (auto &kv : bio.read_map("input_args")) {
+                items_[kv.first] = kv.second;
+                const auto bar = kv.first.find('|');
+                if (bar != std::string::npos) prefixes_.insert(kv.first.substr(0, bar));
+            }
+            set("file", "binary", file_);
+            return true;
+        }
         std::cout << "Reading input parameters from ascii : " << file_ << std::endl;
         std::string line, section;
         prefixes_.clear();

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 
 #include "dynamics.h"
 #include "probgen.h"
@@ -60,6 +61,52 @@ void cycle::base_t::init(const input &args) {
     args("cycle", "t_display_prec", t_display_prec);
     args("cycle", "n_display_freq", n_display_freq);
     if (n_display_freq < 1) n_display_freq = 1;
+    prefix = args.get<std::string>("cycle", "prefix_output", "test");
+    t_output = args.get<double>("cycle", "t_output_next", 0.0);
+    dt_output = args.get<double>("cycle", "dt_output", 1e32);
+    final_output = args.get<bool>("cycle", "final_output", false);
+}
+
+bool cycle::base_t::update_tout() {
+    const double delta_t = t - t_output;
+    if (delta_t < 0) return false;
+    const size_t n_dt = size_t(delta_t / dt_output) + 1;
+    t_output += n_dt * dt_output;
+    return true;
+}
+
+bool cycle::base_t::output_enabled() const {
+    return !std::isnan(t_output + dt_output) && !std::isinf(t_output + dt_output);
+}
+
+std::string cycle::base_t::file_name() {
+    std::stringstream ss;
+    ss << prefix << "_" << std::setfill('0') << std::setw(5) << i_output << ".bin";
+    std::cout << "Making output at cycle = " << i_cycle << ", t = " << t
+              << "; file name: " << ss.str() << std::endl;
+    return ss.str();
+}
+
+// cycle.cpp:273-297: "time", "dt" (the device dt seed), "cycle", "i_out",
+// "t_out"; saving counts the output first
+void cycle::base_t::save(binary_io::base_t &bio, double dt_seed) {
+    ++i_output;
+    const type::float_t dt = (type::float_t)dt_seed;
+    bio.write_value(t, "time");
+    bio.write_value(dt, "dt");
+    bio.write_value(i_cycle, "cycle");
+    bio.write_value(i_output, "i_out");
+    bio.write_value(t_output, "t_out");
+}
+
+void cycle::base_t::load(const binary_io::base_t &bio, double &dt_seed) {
+    type::float_t dt = 0;
+    bio.read(&t, 1, "time");
+    bio.read(&dt, 1, "dt");
+    bio.read(&i_cycle, 1, "cycle");
+    bio.read(&i_output, 1, "i_out");
+    bio.read(&t_output, 1, "t_out");
+    dt_seed = dt;
 }
 
 // ------------------------------------------------------------------ meshgen
@@ -98,9 +145,17 @@ void mesh_t::init(const input &args) {
     if (!p_cyc) enroll_cycle<cycle::base_t>();
     if (!p_com) enroll_comm<comm::base_t>();
     for (auto &r : reads_) r(args);
-    if (args.found("file", "binary"))
-        throw std::runtime_error("restart from a binary snapshot is not supported by this build");
+    input_args_ = args.get_item_map();
+    restart_ = args.found("file", "binary");
     p_cyc->init(args);
+    if (restart_) {
+        restart_file_ = args.get<std::string>("file", "binary");
+        binary_io::base_t bio;
+        bio.load(restart_file_);
+        double seed = 0;
+        p_cyc->load(bio, seed);
+        p_cyc->dt_init = seed;                 // the device seed the run stopped with
+    }
     p_mgn->read(args);
     build_blocks();
     for (auto &p : inits_) p.second(args);
@@ -226,8 +281,74 @@ void mesh_t::step() {
     pull_time();
 }
 
+// mesh.cpp:99-111 with the per-block tags of block.cpp:18-38,
+// geometry.cpp:161-178, hyd_data.cpp:49-57 (hydro_cons_*) and
+// fld_data.cpp:87-95 (field_bf_*); slot 0 of the conserved state and the
+// face field with its two ghost layers (zero here: ghosts are not stored).
 void mesh_t::save(const std::string &file_name) {
-    throw std::runtime_error("snapshot output is not supported by this build: " + file_name);
+    binary_io::base_t bio;
+    bio.write_map(input_args_, "input_args");
+    double t, dtl, seed;
+    check(kr_get_time(ctx_, &t, &dtl, &seed), ctx_, "kr_get_time");
+    p_cyc->t = t;
+    p_cyc->save(bio, seed);
+    const int n_gh = 2;
+    for (size_t b = 0; b < blocks_.size(); ++b) {
+        const block_t &k = blocks_[b];
+        auto &d = *duals_[b];
+        d.copy_d2h();
+        const std::string pre = "block_" + std::to_string(k.id) + "|";
+        bio.write(k.reg.reg.x, 3, sizeof(int), pre + "i_logic");
+        bio.write_value(k.reg.level, pre + "level");
+        type::float_t sfc0[3], vol0 = 1;
+        for (int a = 0; a < 3; ++a) {
+            type::float_t s = 1;
+            for (int m = 1; m < 3; ++m) s *= k.geo.dx0[(a + m) % 3];
+            sfc0[a] = s;
+            vol0 *= k.geo.dx0[a];
+        }
+        const bool uniform[3] = {true, true, true};
+        bio.write(k.geo.xf0.x, 3, sizeof(type::float_t), pre + "xf0");
+        bio.write(k.geo.dx0.x, 3, sizeof(type::float_t), pre + "dx0");
+        bio.write(sfc0, 3, sizeof(type::float_t), pre + "sfc0");
+        bio.write_value(vol0, pre + "vol0");
+        bio.write_value(n_gh, pre + "n_gh");
+        bio.write(k.geo.n_ceff.x, 3, sizeof(int), pre + "n_ceff");
+        bio.write(uniform, 3, sizeof(bool), pre + "uniform");
+        const auto &h = d.h();
+        const int n_step = 2, zero[3] = {0, 0, 0};
+        bio.write_value(h.hyd.n_var, pre + "hydro_cons_n_int");
+        bio.write_value(n_step, pre + "hydro_cons_n_step");
+        bio.write(h.hyd.n.x, 3, sizeof(int), pre + "hydro_cons_n_cell");
+        bio.write(zero, 3, sizeof(int), pre + "hydro_cons_n_gh");
+        bio.write(h.hyd.u_data.data(), h.hyd.u_data.size(), sizeof(double), pre + "hydro_cons_field");
+        if (mhd_) {
+            const int three = 3, gh[3] = {n_gh, n_gh, n_gh};
+            bio.write_value(three, pre + "field_bf_n_int");
+            bio.write_value(n_step, pre + "field_bf_n_step");
+            bio.write(h.fld.N.x, 3, sizeof(int), pre + "field_bf_n_cell");
+            bio.write(gh, 3, sizeof(int), pre + "field_bf_n_gh");
+            bio.write(h.fld.bf_data.data(), h.fld.bf_data.size(), sizeof(double),
+                      pre + "field_bf_field");
+        }
+    }
+    bio.save(file_name);
+}
+
+void mesh_t::load_state() {
+    binary_io::base_t bio;
+    bio.load(restart_file_);
+    for (size_t b = 0; b < blocks_.size(); ++b) {
+        auto &d = *duals_[b];
+        auto &h = d.h();
+        const std::string pre = "block_" + std::to_string(blocks_[b].id) + "|";
+        int nv = 0;
+        bio.read(&nv, 1, pre + "hydro_cons_n_int");
+        if (nv != h.hyd.n_var) throw std::runtime_error("snapshot model differs: " + pre);
+        bio.read(h.hyd.u_data.data(), h.hyd.u_data.size(), pre + "hydro_cons_field");
+        if (mhd_) bio.read(h.fld.bf_data.data(), h.fld.bf_data.size(), pre + "field_bf_field");
+        d.copy_h2d();
+    }
 }
 
 // cycle::evolve (cycle.cpp:152-194): the same loop, stop rule and Speed
@@ -242,12 +363,20 @@ void mesh_t::evolve() {
     const auto start = clock::now();
     auto prev = start;
     check(kr_sync(ctx_), ctx_, "kr_sync");
+    // snapshot due before a cycle (cycle.cpp:124-150): with outputs on,
+    // cycles run one per kr_advance so that t is checked before each
+    const bool outputs = c.output_enabled();
+    bool to_stop = false;
     try {
         // the reference's loop runs n_cycle_lim - i_cycle + 1 steps unless
         // t reaches t_lim first; i_cycle stops at n_cycle_lim
         long long left = (long long)c.n_cycle_lim - c.i_cycle + 1;
         while (left > 0) {
-            const int batch = (int)std::min<long long>(left, c.n_display_freq);
+            if (outputs) {
+                pull_time();
+                if (c.update_tout()) save(c.file_name());
+            }
+            const int batch = outputs ? 1 : (int)std::min<long long>(left, c.n_display_freq);
             int done = 0;
             check(kr_advance(ctx_, batch, &done), ctx_, "kr_advance");
             pull_time();
@@ -267,13 +396,16 @@ void mesh_t::evolve() {
         }
     } catch (const std::exception &e) {
         std::cerr << e.what();
+        to_stop = true;
     }
+    if (to_stop && c.final_output) return save(c.file_name());
     check(kr_sync(ctx_), ctx_, "kr_sync");
     const double dwt = std::chrono::duration<double>(clock::now() - start).count();
     const double speed = n_cell_tot / dwt * (c.i_cycle - c.i_cycle_0 + 1);
     if (p_com->is_root())
         std::cout << "Duration = " << dwt << " s, " << "Speed = " << speed << " cell/s,\n"
                   << "Pace = " << 1 / speed << " s/cell.\n";
+    if (c.final_output && !std::isnan(c.t_output)) save(c.file_name());
 }
 
 }  // namespace mesh

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "binary_io.h"
 #include "input.h"
 #include "kratos_amd.h"
 #include "types.h"
@@ -43,13 +44,6 @@ struct base_t {
 // The device type a generator enrols (device/general.h): the MI355X
 // context behind the C-ABI.
 struct device_t : device::base_t {};
-
-namespace binary_io {
-struct base_t {
-    virtual ~base_t() {}
-};
-typedef base_t default_t;
-}  // namespace binary_io
 
 namespace mesh {
 
@@ -147,13 +141,24 @@ private:
 using block::block_t;
 
 namespace cycle {
-// Time loop state and keys of mesh/cycle/cycle.cpp:20-65.
+// Time loop state and keys of mesh/cycle/cycle.cpp:20-65, with the
+// snapshot schedule (:53-60, :228-237, :256-297).
 struct base_t {
     virtual ~base_t() {}
     double t = 0, t_lim = 3.40282346638528859812e+38, dt_init = 0, dt_expand = 2;
     int i_cycle = 0, i_cycle_0 = 0, n_cycle_lim = 0, n_display_freq = 1, t_display_prec = 6;
     bool top_up = false;
+    std::string prefix = "test";
+    double t_output = 0, dt_output = 1e32;
+    bool final_output = false;
+    int i_output = 0;
     virtual void init(const input &args);
+    // output due at the current t (advances t_output past t), cycle.cpp:228-237
+    bool update_tout();
+    bool output_enabled() const;
+    std::string file_name();
+    void save(binary_io::base_t &bio, double dt_seed);
+    void load(const binary_io::base_t &bio, double &dt_seed);
 };
 }  // namespace cycle
 
@@ -240,7 +245,12 @@ public:
     virtual void init(const input &args);
     virtual void step();
     virtual void evolve();
+    // Snapshot of every block's slot 0, the time and the parameters in the
+    // reference's binary container (mesh/mesh.cpp:99-111); restart by
+    // passing the snapshot as a parameter file ([file] binary).
     virtual void save(const std::string &file_name);
+    bool is_restart() const { return restart_; }
+    void load_state();                     // blocks' state from the restart snapshot
 
     // blocks of this process, in the generator's order
     std::vector<block_t>::iterator begin() { return blocks_.begin(); }
@@ -276,6 +286,9 @@ private:
     kr_ctx *ctx_ = nullptr;
     bool mhd_ = false;
     double dt_last_ = 0;
+    bool restart_ = false;
+    std::string restart_file_;
+    std::map<std::string, std::string> input_args_;
 };
 
 // Throws std::runtime_error carrying the C-ABI message when rc != KR_OK.

@@ -88,3 +88,48 @@ def test_vortex_generator(gpu_lib, tmp_path):
         np.testing.assert_array_equal(x, y)
     divb = float(out.split("max |div B| =")[1].split()[0])
     assert divb < 1e-12
+
+
+def _snap_fields(path):
+    from test_host_api import parse_snapshot
+    tags = parse_snapshot(path)
+    f = {k: v[1] for k, v in tags.items()}
+    return f
+
+
+def test_vortex_snapshot_restart(gpu_lib, tmp_path):
+    """Snapshots in the reference's container: a run stopped after 5 cycles
+    and restarted from its snapshot (the snapshot as the parameter file)
+    ends bit-identical to an uninterrupted run of the same 13 cycles.  (The
+    evolve loop's rule: a run stopped by n_cycle_lim = 4 has taken 5 cycles
+    and stores cycle = 4; the restart to n_cycle_lim = 11 takes 8 more.)"""
+    exe = os.path.join(HOST, "bin", "vortex")
+    par = os.path.join(HOST, "usr_example", "vortex", "vortex.par")
+
+    def run(args, over):
+        o = tmp_path / f"over_{len(list(tmp_path.iterdir()))}.par"
+        o.write_text(over)
+        print(f"\n[case] host pgen vortex {args} {over!r}", flush=True)
+        r = subprocess.run(["timeout", "-k", "10", "120", exe] + args + [str(o)],
+                           capture_output=True, text=True, cwd=tmp_path)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0, r.stderr
+
+    run([par], f"[cycle]\nn_cycle_lim = 12\nfinal_output = 1\nprefix_output = {tmp_path}/A\n")
+    run([par], f"[cycle]\nn_cycle_lim = 4\nfinal_output = 1\nprefix_output = {tmp_path}/B\n")
+    b1 = tmp_path / "B_00001.bin"
+    run([str(b1)], f"[cycle]\nn_cycle_lim = 11\nprefix_output = {tmp_path}/C\n")
+    a, c = _snap_fields(tmp_path / "A_00001.bin"), _snap_fields(tmp_path / "C_00002.bin")
+    assert np.frombuffer(a["cycle"], "<i4")[0] == 12
+    assert np.frombuffer(c["cycle"], "<i4")[0] == 11
+    assert a["time"] == c["time"] and a["dt"] == c["dt"]
+    blocks = sorted(k for k in a if k.endswith("|hydro_cons_field"))
+    assert len(blocks) == 8
+    for k in blocks + [k.replace("hydro_cons", "field_bf") for k in blocks]:
+        np.testing.assert_array_equal(np.frombuffer(a[k], "<f8"), np.frombuffer(c[k], "<f8"))
+    # the initial snapshot holds the initial condition and the parameters
+    s0 = _snap_fields(tmp_path / "A_00000.bin")
+    assert np.frombuffer(s0["time"], "<f8")[0] == 0.0
+    from test_host_api import parse_args_map
+    args0 = parse_args_map(s0["input_args"])
+    assert args0["mesh|n_cell_global"].split() == ["32", "32", "32"]
