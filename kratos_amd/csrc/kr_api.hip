@@ -106,7 +106,7 @@ struct SweepVariant {
 };
 constexpr SweepVariant kSweepVariants[] = {
     {32, 8, 512, 1}, {32, 8, 512, 4}, {16, 16, 512, 1}, {16, 16, 512, 4},
-    {16, 8, 256, 1}, {8, 8, 192, 1},
+    {16, 8, 256, 1}, {8, 8, 192, 1}, {16, 8, 256, 4},
 };
 constexpr int kNumSweepVariants = sizeof(kSweepVariants) / sizeof(kSweepVariants[0]);
 
@@ -362,7 +362,8 @@ struct Launcher {
         case 2: sweep_t<FT, 16, 16, 512, 1>(m, step); break;
         case 3: sweep_t<FT, 16, 16, 512, f32 ? 4 : 1>(m, step); break;
         case 4: sweep_t<FT, 16, 8, 256, 1>(m, step); break;
-        default: sweep_t<FT, 8, 8, 192, 1>(m, step); break;
+        case 5: sweep_t<FT, 8, 8, 192, 1>(m, step); break;
+        default: sweep_t<FT, 16, 8, 256, f32 ? 4 : 1>(m, step); break;
         }
     }
 

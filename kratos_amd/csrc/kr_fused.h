@@ -164,15 +164,19 @@ k_sweep(MeshView<FT> m, int step) {
     const FT gam = (FT)m.gamma;
     const size_t nc = m.ncell;
 
-    // W ring rows are padded to whole 64-lane dword chunks: the planes are
-    // brought in by global_load_lds (lane i of a chunk writes dword i), so
-    // every chunk a wave writes lies inside its row
+    // W ring: the planes are brought in by global_load_lds in 64-lane dword
+    // chunks (lane i of a chunk writes dword i of the chunk); lanes past the
+    // end of a row issue no load, so nothing is written outside the row
     constexpr int DPC = (int)sizeof(FT) / 4;                       // dwords per column
-    constexpr int NCHUNK = (S::NCOL * DPC + 63) / 64;              // chunks per row
-    constexpr int NCOLP = NCHUNK * 64 / DPC;                       // padded columns
-    __shared__ FT sW[3][8][NCOLP];
-    __shared__ FT sFe[2][3][S::NF];     // rho, 2 EMF slots of x / y faces
-    __shared__ FT sFh[5][S::NF];        // hydro fluxes of the current plane
+    constexpr int NDW = S::NCOL * DPC;                             // dwords per row
+    constexpr int NCHUNK = (NDW + 63) / 64;                        // chunks per row
+    __shared__ FT sW[3][8][S::NCOL];
+    // in-plane faces: mass flux and the EMF slot the next plane's edges
+    // need (E_y of x faces, E_x of y faces) for two planes; E_z and the
+    // energy / momentum fluxes of the current plane only
+    __shared__ FT sFe[2][2][S::NF];
+    __shared__ FT sFez[S::NF];
+    __shared__ FT sFh[4][S::NF];
     __shared__ FT sFz[3][S::NZG];       // rho, E_x, E_y of the current z face
     __shared__ FT sEz[2][S::NEZ];
     __shared__ FT sEx[2][S::NEX];
@@ -204,16 +208,15 @@ k_sweep(MeshView<FT> m, int step) {
     // Asynchronous load of plane z of W into ring slot `slot` (no staging
     // registers).  Wave w fills chunks w, w + NT/64, ... of every variable
     // row; lane l of chunk k carries dword k*64 + l of the row, i.e. dword
-    // `half` of column col.  Padding lanes (col >= NCOL) read column 0 and
-    // write the row's padding, which nothing reads.
+    // `half` of column col.
     auto issue_w = [&](int z, int slot) {
         const int wave = tid >> 6, lane = tid & 63;
         for (int k = wave; k < NCHUNK; k += NT / 64) {
             const int d = k * 64 + lane;
+            if (d >= NDW) continue;                 // past the row: no load, no write
             const int col = d / DPC;
             const int half = d - col * DPC;
-            const int cc = col < S::NCOL ? col : 0;
-            const FT *src = wsrc(cc % S::HX - 2, cc / S::HX - 2, z);
+            const FT *src = wsrc(col % S::HX - 2, col / S::HX - 2, z);
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const char *gp = (const char *)(src + (size_t)v * nc) + 4 * half;
@@ -386,17 +389,16 @@ k_sweep(MeshView<FT> m, int step) {
                 else
                     hlld<FT>(flx, wl, wr, gam);
                 // back to the mesh frame
-                sFh[0][it] = flx[RHO];
-                sFh[1][it] = flx[ENE];
+                sFh[0][it] = flx[ENE];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) sFh[2 + (a + k) % 3][it] = flx[MOM + k];
+                for (int k = 0; k < 3; ++k) sFh[1 + (a + k) % 3][it] = flx[MOM + k];
                 sFe[fe][0][it] = flx[RHO];
                 if (xf) {                        // x faces: E_y, E_z
                     sFe[fe][1][it] = flx[BCC + 2];
-                    sFe[fe][2][it] = -flx[BCC + 1];
+                    sFez[it] = -flx[BCC + 1];
                 } else {                         // y faces: E_x, E_z
                     sFe[fe][1][it] = -flx[BCC + 1];
-                    sFe[fe][2][it] = flx[BCC + 2];
+                    sFez[it] = flx[BCC + 2];
                 }
             }
         }
@@ -424,9 +426,9 @@ k_sweep(MeshView<FT> m, int step) {
                     // ee_z on plane p: a1 = x, a2 = y
                     const FT e = edge_emf<FT>(
                         ec(s0, x, y, 2), ec(s0, x - 1, y, 2), ec(s0, x - 1, y - 1, 2), ec(s0, x, y - 1, 2),
-                        sFe[fe][2][S::fxi(x, y - 1)], sFe[fe][2][S::fxi(x, y)],
+                        sFez[S::fxi(x, y - 1)], sFez[S::fxi(x, y)],
                         sFe[fe][0][S::fyi(x, y)], sFe[fe][0][S::fyi(x - 1, y)],
-                        sFe[fe][2][S::fyi(x - 1, y)], sFe[fe][2][S::fyi(x, y)],
+                        sFez[S::fyi(x - 1, y)], sFez[S::fyi(x, y)],
                         sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
                     sEz[fe][S::ezi(x, y)] = e;
                 } else if (it < S::NEZ + S::NEX) {
@@ -508,8 +510,10 @@ k_sweep(MeshView<FT> m, int step) {
 #pragma unroll
                             for (int v = 0; v < 5; ++v) {
                                 double res(0);
-                                if (fa > 0) res += fa * sFh[v][i0];
-                                if (fa > 0) res -= fa * sFh[v][i1];
+                                const FT f0 = v == 0 ? sFe[fq][0][i0] : sFh[v - 1][i0];
+                                const FT f1 = v == 0 ? sFe[fq][0][i1] : sFh[v - 1][i1];
+                                if (fa > 0) res += fa * f0;
+                                if (fa > 0) res -= fa * f1;
                                 uu[v] += res;
                             }
                         } else {
@@ -517,8 +521,10 @@ k_sweep(MeshView<FT> m, int step) {
 #pragma unroll
                             for (int v = 0; v < 5; ++v) {
                                 double res(0);
-                                if (fa > 0) res += (double)(fa * sFh[v][i0]);
-                                if (fa > 0) res -= (double)(fa * sFh[v][i1]);
+                                const FT f0 = v == 0 ? sFe[fq][0][i0] : sFh[v - 1][i0];
+                                const FT f1 = v == 0 ? sFe[fq][0][i1] : sFh[v - 1][i1];
+                                if (fa > 0) res += (double)(fa * f0);
+                                if (fa > 0) res -= (double)(fa * f1);
                                 uu[v] += res;
                             }
                         }

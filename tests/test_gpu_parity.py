@@ -152,7 +152,7 @@ def _oracle_state(N, blk, n_cycles, seed):
 
 
 @pytest.mark.parametrize("tile", [(32, 8, 1), (32, 8, 4), (16, 16, 1), (16, 16, 4), (16, 8, 1),
-                                  (8, 8, 1)])
+                                  (16, 8, 4), (8, 8, 1)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_fused_tile_variants(gpu_lib, tile, precision):
     """Every fused-kernel variant the autotuner may pick, forced one at a
