@@ -179,6 +179,149 @@ class MeshSpec:
         return u, bfs
 
 
+class RefinedMeshSpec:
+    """A statically refined block mesh (SMR): blocks of one size at several
+    levels, the reference's tree_t built from ``[refine_region_*]`` entries.
+
+    ``refine`` is a sequence of ``(level, x_min, x_max)`` boxes; every block
+    of that level whose index lies in the box (inclusive,
+    meshgen.cpp:111-158 with ``search_blk_idx`` at :355-374) is inserted
+    with tree_t::insert (tree.cpp:185-205): its parent is masked, the
+    parent's children and the parent's 26 same-level neighbours are
+    inserted too, which keeps the mesh 2:1 balanced across faces, edges and
+    corners.  ``fill_base_level`` (tree.cpp:207-218) completes level 0.
+    Blocks are the unmasked tree nodes in the tree's order: level, then
+    x, y, z of the logical index (region.h lex_ord_t).
+    """
+
+    def __init__(self, n_cell_global, n_cell_block, refine=(), x_min=(0.0, 0.0, 0.0),
+                 x_max=(1.0, 1.0, 1.0), bnd=("per",) * 6):
+        self.n_cell_global = tuple(int(v) for v in n_cell_global)
+        self.n_cell_block = tuple(int(v) for v in n_cell_block)
+        self.x_min, self.x_max, self.bnd = tuple(x_min), tuple(x_max), tuple(bnd)
+        for a in range(3):
+            if self.n_cell_global[a] % self.n_cell_block[a]:
+                raise ValueError("Mesh size indivisible by sub-mesh.")
+        self.nblk = tuple(self.n_cell_global[a] // self.n_cell_block[a] for a in range(3))
+        self.periodic = tuple(self.bnd[2 * a] == "per" for a in range(3))
+        self.refine = [(int(l), tuple(lo), tuple(hi)) for l, lo, hi in refine]
+        tree = {}
+
+        def lim(level, a):
+            return self.nblk[a] << level
+
+        def regularize(level, r):
+            return tuple((r[a] + lim(level, a)) % lim(level, a) if self.periodic[a] else r[a]
+                         for a in range(3))
+
+        def inside(level, r):
+            return all(0 <= r[a] < lim(level, a) for a in range(3))
+
+        def insert(level, r):
+            if (level,) + r in tree:
+                return
+            if level > 0:
+                rc = tuple(v // 2 for v in r)
+                insert(level - 1, rc)
+                tree[(level - 1,) + rc] = True
+                for c in range(8):
+                    tree.setdefault((level,) + tuple(2 * rc[a] + ((c >> a) & 1) for a in range(3)),
+                                    False)
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            if dx == dy == dz == 0:
+                                continue
+                            q = regularize(level - 1, (rc[0] + dx, rc[1] + dy, rc[2] + dz))
+                            if inside(level - 1, q):
+                                insert(level - 1, q)
+            tree.setdefault((level,) + r, False)
+
+        for level, lo, hi in self.refine:
+            if level <= 0:
+                continue
+            i0 = [self.search_blk_idx(lo[a], level, a) for a in range(3)]
+            i1 = [self.search_blk_idx(hi[a], level, a) for a in range(3)]
+            if min(i0 + i1) < 0:
+                raise ValueError("Incorrect SMR region")
+            for z in range(i0[2], i1[2] + 1):
+                for y in range(i0[1], i1[1] + 1):
+                    for x in range(i0[0], i1[0] + 1):
+                        insert(level, (x, y, z))
+        for z in range(self.nblk[2]):
+            for y in range(self.nblk[1]):
+                for x in range(self.nblk[0]):
+                    tree.setdefault((0, x, y, z), False)
+        self.leaves = [k for k in sorted(tree) if not tree[k]]
+        self.n_levels = 1 + max(k[0] for k in self.leaves)
+
+    def location(self, x_logic, a):
+        return self.x_min[a] + x_logic * (self.x_max[a] - self.x_min[a])
+
+    def search_blk_idx(self, loc, lvl, a):
+        """Block index of level `lvl` along axis `a` containing `loc`."""
+        if loc > self.x_max[a] or loc < self.x_min[a]:
+            return -1
+        i_tot = self.nblk[a] << lvl
+        i_l, i_r = 0, i_tot
+        while i_r - i_l > 1:
+            i_m = (i_l + i_r) // 2
+            if self.location(i_m / float(i_tot), a) < loc:
+                i_l = i_m
+            else:
+                i_r = i_m
+        return i_l
+
+    @property
+    def n_blocks(self):
+        return len(self.leaves)
+
+    @property
+    def n_cells(self):
+        n = self.n_cell_block
+        return self.n_blocks * n[0] * n[1] * n[2]
+
+    def level(self, b):
+        return self.leaves[b][0]
+
+    def reg(self, b):
+        return self.leaves[b][1:]
+
+    def geometry(self, b):
+        """(xf0, dx0) of block b (meshgen.cpp:286-321 at the block's level)."""
+        level, r = self.level(b), self.reg(b)
+        xf0, dx0 = [0.0] * 3, [0.0] * 3
+        for a in range(3):
+            n = self.n_cell_block[a]
+
+            def loc(i, a=a, n=n):
+                i = (r[a] + i / n) / float(self.nblk[a] << level)
+                return self.location(i, a)
+
+            xf0[a] = loc(0.0)
+            dx0[a] = loc(1.0) - xf0[a]
+        return xf0, dx0
+
+    def block_descs(self, members=1):
+        if members != 1:
+            raise ValueError("refined meshes have one member")
+        arr = (_capi.KrBlockDesc * self.n_blocks)()
+        for b in range(self.n_blocks):
+            d = arr[b]
+            d.id, d.level, d.group = b, self.level(b), 0
+            xf0, dx0 = self.geometry(b)
+            for a in range(3):
+                d.reg[a] = self.reg(b)[a]
+                d.n_ceff[a] = self.n_cell_block[a]
+                d.xf0[a], d.dx0[a] = xf0[a], dx0[a]
+            for k in range(27):
+                d.nbr[k] = -1
+        return arr
+
+    def split(self, u, bfs=None):
+        raise TypeError("refined meshes take per-block arrays (upload_blocks)")
+
+
 @dataclass
 class Physics:
     mhd: bool = True
@@ -265,6 +408,30 @@ class Context:
                         self.h, "kr_download")
             blocks.append((ua, ba))
         return self.spec.gather(blocks, self.mhd)
+
+    def upload_blocks(self, blocks):
+        """Per-block (u_aos, bf_aos) in the reference's host layout, in
+        block order (the only way to fill a refined mesh)."""
+        for b, (ua, ba) in enumerate(blocks):
+            ua = np.ascontiguousarray(ua, dtype=np.float64)
+            bp = None
+            if self.mhd:
+                ba = np.ascontiguousarray(ba, dtype=np.float64)
+                bp = ba.ctypes.data
+            _capi.check(self.lib.kr_upload(self.h, b, ua.ctypes.data, bp), self.h, "kr_upload")
+
+    def download_blocks(self):
+        n = self.spec.n_cell_block
+        nvar = 8 if self.mhd else 5
+        out = []
+        for b in range(self.spec.n_blocks):
+            ua = np.zeros(n[0] * n[1] * n[2] * nvar)
+            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            _capi.check(self.lib.kr_download(self.h, b, ua.ctypes.data,
+                                             ba.ctypes.data if ba is not None else None),
+                        self.h, "kr_download")
+            out.append((ua, ba))
+        return out
 
     def step(self):
         dt = C.c_double()

@@ -179,3 +179,149 @@ def blast(N, gamma=5.0 / 3.0, rho0=1.0, p0=0.1, p_in=10.0, r_in=0.1,
     u[0] = rho
     u[1] = total_energy(rho, pre, [0 * rho] * 3, cell_centred_b(bx, by, bz), gamma)
     return u, (bx, by, bz)
+
+
+# ----------------------------------------------------------------------
+# Per-block initial states (refined meshes have no global array): each
+# generator evaluates its fields at the block's own cell and face centres
+# and returns the reference's host AoS arrays of every block.
+def _pack_block(n, rho, pre, vel, bfs, gamma):
+    """(u_aos, bf_aos) of one block from cell fields (nz, ny, nx) and, for
+    MHD, face fields bx (nz, ny, nx+1), by (nz, ny+1, nx), bz (nz+1, ny, nx)."""
+    mhd = bfs is not None
+    nvar = 8 if mhd else 5
+    ub = np.zeros((nvar, n[2], n[1], n[0]))
+    ub[0] = rho
+    for a in range(3):
+        ub[2 + a] = rho * vel[a]
+    bcc = cell_centred_b(*bfs) if mhd else [0 * rho] * 3
+    ub[1] = total_energy(rho, pre, vel, bcc, gamma)
+    bf_aos = None
+    if mhd:
+        for a in range(3):
+            ub[5 + a] = bcc[a]
+        f = np.zeros((n[2] + 5, n[1] + 5, n[0] + 5, 3))
+        f[2:n[2] + 2, 2:n[1] + 2, 2:n[0] + 3, 0] = bfs[0]
+        f[2:n[2] + 2, 2:n[1] + 3, 2:n[0] + 2, 1] = bfs[1]
+        f[2:n[2] + 3, 2:n[1] + 2, 2:n[0] + 2, 2] = bfs[2]
+        bf_aos = f.reshape(-1)
+    return np.ascontiguousarray(np.moveaxis(ub, 0, -1)).reshape(-1), bf_aos
+
+
+def _block_axes(spec, b):
+    """Cell-centre and face coordinates of block b along each axis."""
+    xf0, dx0 = spec.geometry(b)
+    n = spec.n_cell_block
+    cc = [xf0[a] + (np.arange(n[a]) + 0.5) * dx0[a] for a in range(3)]
+    fc = [xf0[a] + np.arange(n[a] + 1) * dx0[a] for a in range(3)]
+    return cc, fc, dx0
+
+
+def blast_blocks(spec, gamma=5.0 / 3.0, rho0=1.0, p0=0.1, p_in=10.0, r_in=0.1,
+                 b0=(1.0 / math.sqrt(2.0), 1.0 / math.sqrt(2.0), 0.0), mhd=True):
+    """MHD blast on every block of `spec`: uniform rho and oblique B, p = p_in
+    inside radius r_in around the origin, p0 outside."""
+    n = spec.n_cell_block
+    out = []
+    for b in range(spec.n_blocks):
+        cc, _, _ = _block_axes(spec, b)
+        r2 = cc[2][:, None, None] ** 2 + cc[1][None, :, None] ** 2 + cc[0][None, None, :] ** 2
+        pre = np.where(r2 < r_in * r_in, p_in, p0)
+        rho = np.full((n[2], n[1], n[0]), rho0)
+        vel = [0 * rho] * 3
+        bfs = None
+        if mhd:
+            bfs = (np.full((n[2], n[1], n[0] + 1), b0[0]), np.full((n[2], n[1] + 1, n[0]), b0[1]),
+                   np.full((n[2] + 1, n[1], n[0]), b0[2]))
+        out.append(_pack_block(n, rho, pre, vel, bfs, gamma))
+    return out
+
+
+def smooth_blocks(spec, seed=0, gamma=5.0 / 3.0, mhd=True, amp=0.1, mach=0.3, b_amp=0.04,
+                  b0=(0.4, -0.3, 0.2)):
+    """Smooth periodic state on every block of `spec`: a few seeded Fourier
+    modes (integer wavenumbers on the box) in rho, p and v, and for MHD a
+    uniform field plus the discrete curl of a seeded edge vector potential,
+    so div B = 0 in every block at its own resolution."""
+    rng = SplitMix64(seed)
+    L = [spec.x_max[a] - spec.x_min[a] for a in range(3)]
+
+    def modes(k):
+        return [(tuple(int(rng.uniform() * 3) + 1 for _ in range(3)),
+                 2 * math.pi * rng.uniform(), rng.uniform() - 0.5) for _ in range(k)]
+
+    def field(ms, x, y, z):
+        s = 0.0
+        for kv, ph, am in ms:
+            s = s + am * np.sin(2 * math.pi * (kv[0] * (x - spec.x_min[0]) / L[0]
+                                                + kv[1] * (y - spec.x_min[1]) / L[1]
+                                                + kv[2] * (z - spec.x_min[2]) / L[2]) + ph)
+        return s
+
+    m_rho, m_pre = modes(3), modes(3)
+    m_v = [modes(3) for _ in range(3)]
+    m_a = [modes(2) for _ in range(3)]
+    n = spec.n_cell_block
+    out = []
+    for b in range(spec.n_blocks):
+        cc, fc, dx = _block_axes(spec, b)
+        Z, Y, X = np.meshgrid(cc[2], cc[1], cc[0], indexing="ij")
+        rho = 1.0 + amp * field(m_rho, X, Y, Z)
+        pre = 1.0 / gamma + amp / gamma * field(m_pre, X, Y, Z)
+        vel = [mach * field(m_v[a], X, Y, Z) for a in range(3)]
+        bfs = None
+        if mhd:
+            # A_a on the edges along a: cell centre along a, faces across
+            Ax = b_amp * field(m_a[0], *_grid(cc[0], fc[1], fc[2]))
+            Ay = b_amp * field(m_a[1], *_grid(fc[0], cc[1], fc[2]))
+            Az = b_amp * field(m_a[2], *_grid(fc[0], fc[1], cc[2]))
+            # arrays are (z, y, x); B = curl A on the faces
+            bx = (Az[:, 1:, :] - Az[:, :-1, :]) / dx[1] - (Ay[1:, :, :] - Ay[:-1, :, :]) / dx[2]
+            by = (Ax[1:, :, :] - Ax[:-1, :, :]) / dx[2] - (Az[:, :, 1:] - Az[:, :, :-1]) / dx[0]
+            bz = (Ay[:, :, 1:] - Ay[:, :, :-1]) / dx[0] - (Ax[:, 1:, :] - Ax[:, :-1, :]) / dx[1]
+            bfs = (bx + b0[0], by + b0[1], bz + b0[2])
+        out.append(_pack_block(n, rho, pre, vel, bfs, gamma))
+    return out
+
+
+def _grid(xs, ys, zs):
+    """(x, y, z) coordinate arrays of shape (len(zs), len(ys), len(xs))."""
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    return X, Y, Z
+
+
+def wave_blocks(spec, amp=0.1, vel=(1.0, 0.0, 0.0), pre=1.0, gamma=5.0 / 3.0, mhd=False,
+                b0=(0.0, 0.0, 0.0)):
+    """Density wave rho = 1 + amp sin(2 pi (x - x_min) / L_x) in a uniform flow
+    `vel` at uniform pressure (and uniform B for MHD): the exact solution is
+    the translated profile.  Cells hold the exact cell average of rho."""
+    n = spec.n_cell_block
+    L = spec.x_max[0] - spec.x_min[0]
+    out = []
+    for b in range(spec.n_blocks):
+        _, fc, dx = _block_axes(spec, b)
+        xl = (fc[0][:-1] - spec.x_min[0]) / L
+        xr = (fc[0][1:] - spec.x_min[0]) / L
+        avg = 1.0 + amp * (np.cos(2 * math.pi * xl) - np.cos(2 * math.pi * xr)) / (
+            2 * math.pi * (xr - xl))
+        rho = np.broadcast_to(avg[None, None, :], (n[2], n[1], n[0])).copy()
+        p = np.full_like(rho, pre)
+        v = [np.full_like(rho, vel[a]) for a in range(3)]
+        bfs = None
+        if mhd:
+            bfs = (np.full((n[2], n[1], n[0] + 1), b0[0]), np.full((n[2], n[1] + 1, n[0]), b0[1]),
+                   np.full((n[2] + 1, n[1], n[0]), b0[2]))
+        out.append(_pack_block(n, rho, p, v, bfs, gamma))
+    return out
+
+
+def wave_exact(spec, b, t, amp=0.1, vel=1.0):
+    """Exact cell averages of the translated density wave on block b at time t."""
+    n = spec.n_cell_block
+    L = spec.x_max[0] - spec.x_min[0]
+    _, fc, _ = _block_axes(spec, b)
+    xl = (fc[0][:-1] - spec.x_min[0] - vel * t) / L
+    xr = (fc[0][1:] - spec.x_min[0] - vel * t) / L
+    avg = 1.0 + amp * (np.cos(2 * math.pi * xl) - np.cos(2 * math.pi * xr)) / (
+        2 * math.pi * (xr - xl))
+    return np.broadcast_to(avg[None, None, :], (n[2], n[1], n[0]))

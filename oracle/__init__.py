@@ -53,6 +53,9 @@ def load():
     D = C.POINTER(C.c_double)
     lib.kro_create.restype = C.c_void_p
     lib.kro_create.argtypes = [C.POINTER(KroConfig)]
+    lib.kro_create_blocks.restype = C.c_void_p
+    lib.kro_create_blocks.argtypes = [C.POINTER(KroConfig), C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
     lib.kro_destroy.argtypes = [C.c_void_p]
     lib.kro_last_error.restype = C.c_char_p
     lib.kro_n_blocks.argtypes = [C.c_void_p]
@@ -100,7 +103,13 @@ class OracleMesh:
         c.n_threads = int(n_threads)
         if phys.t_0 != 0.0:
             raise ValueError("oracle starts at t = 0")
-        h = lib.kro_create(C.byref(c))
+        if hasattr(spec, "leaves"):             # statically refined mesh
+            nb = spec.n_blocks
+            lv = (C.c_int * nb)(*[spec.level(b) for b in range(nb)])
+            rg = (C.c_int * (3 * nb))(*[v for b in range(nb) for v in spec.reg(b)])
+            h = lib.kro_create_blocks(C.byref(c), nb, lv, rg)
+        else:
+            h = lib.kro_create(C.byref(c))
         if not h:
             raise RuntimeError("kro_create: " + lib.kro_last_error().decode())
         self.h = h
@@ -143,6 +152,28 @@ class OracleMesh:
                                    ba.ctypes.data if ba is not None else None)
             blocks.append((ua, ba))
         return self.spec.gather(blocks, self.mhd)
+
+    def upload_blocks(self, blocks):
+        for b, (ua, ba) in enumerate(blocks):
+            ua = np.ascontiguousarray(ua, dtype=np.float64)
+            bp = None
+            if self.mhd:
+                ba = np.ascontiguousarray(ba, dtype=np.float64)
+                bp = ba.ctypes.data
+            if self.lib.kro_set_block(self.h, b, ua.ctypes.data, bp) != 0:
+                raise RuntimeError("kro_set_block failed")
+
+    def download_blocks(self):
+        n = self.spec.n_cell_block
+        nvar = 8 if self.mhd else 5
+        out = []
+        for b in range(self.spec.n_blocks):
+            ua = np.zeros(n[0] * n[1] * n[2] * nvar)
+            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            self.lib.kro_get_block(self.h, b, ua.ctypes.data,
+                                   ba.ctypes.data if ba is not None else None)
+            out.append((ua, ba))
+        return out
 
     def advance(self, n_cycles):
         r = self.lib.kro_advance(self.h, int(n_cycles))

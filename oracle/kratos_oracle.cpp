@@ -40,6 +40,9 @@ inline double dmin(double a, double b) { return a < b ? a : b; } // functions.h:
 template <class T> inline T tmin(T a, T b) { return a < b ? a : b; } // functions.h:26-30
 template <class T> inline T tmax(T a, T b) { return a > b ? a : b; } // functions.h:20-24
 inline double tabs(double a) { return a > 0 ? a : -a; }  // functions.h:32-36
+inline double minmod(double a, double b) {                // functions.h:159-170
+    return ((a < 0) ^ (b < 0)) ? 0 : (std::fabs(a) < std::fabs(b) ? a : b);
+}
 inline int sgn(double a) { return a > 0 ? 1 : (a < 0 ? -1 : 0); } // functions.h:194-198
 
 // utils::idx_swap (utilities/mapping/mapping.h:225-237)
@@ -94,13 +97,20 @@ struct Cfg {
 
 struct Block {
     int reg[3];
+    int level = 0;
     int n[3];
     int ndim;
     int gw[3];                 // ghosts of w along each axis
     double xf0[3], dx0[3], sfc0[3], vol0;
     bool phy[3][2];
+    // SMR: level of the leaf covering the neighbour region in direction
+    // ((dz+1)*3 + (dy+1))*3 + (dx+1), relative to this block (-1 coarser,
+    // 0 same, +1 finer; kOut outside a non-periodic domain).
+    int lev_nb[27];
     Arr u, w, f[3], bf, ec, ee;
 };
+constexpr int kOut = 9;
+inline int dir_index(const int d[3]) { return ((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1); }
 
 /* ------------------------------------------------------------------ */
 /* Equation of state (hydro/eos/adiabatic.h)                           */
@@ -489,8 +499,7 @@ struct Mesh {
         for (auto &x : th) x.join();
     }
 
-    void build() {
-        const int nv = nvar();
+    void build(const std::vector<int> *levels = nullptr, const std::vector<int> *regs = nullptr) {
         ndim = 0;                                  // meshgen.cpp:23-27
         for (int a = 0; a < 3; ++a)
             if (cfg.ng[a] > 1) ndim = a + 1;
@@ -503,65 +512,446 @@ struct Mesh {
             for (int a = 0; a < 3; ++a)
                 if (cfg.nb[a] < 2)
                     throw std::runtime_error("oracle MHD needs 3-D blocks");
-        blocks.resize((size_t)nblk[0] * nblk[1] * nblk[2]);
-        for (int bz = 0; bz < nblk[2]; ++bz)
-            for (int by = 0; by < nblk[1]; ++by)
-                for (int bx = 0; bx < nblk[0]; ++bx) {
-                    const int r[3] = {bx, by, bz};
-                    Block &b = blocks[bid(r)];
-                    b.ndim = 0;
-                    for (int a = 0; a < 3; ++a) {
-                        b.reg[a] = r[a];
-                        b.n[a] = cfg.nb[a];
-                        b.ndim += (b.n[a] > 1 ? 1 : 0);     // geometry.cpp:46
-                        b.gw[a] = b.n[a] > 1 ? 2 : 0;       // hyd_data.cpp:21-24
+        if (!levels) {
+            blocks.resize((size_t)nblk[0] * nblk[1] * nblk[2]);
+            for (int bz = 0; bz < nblk[2]; ++bz)
+                for (int by = 0; by < nblk[1]; ++by)
+                    for (int bx = 0; bx < nblk[0]; ++bx) {
+                        const int r[3] = {bx, by, bz};
+                        init_block(blocks[bid(r)], 0, r);
                     }
-                    // meshgen.cpp:286-321 (uniform: loc(0), loc(1) - loc(0))
-                    for (int a = 0; a < 3; ++a) {
-                        auto loc = [&](double i) -> double {
-                            if (b.n[a] == 1) return i;
-                            i = (b.reg[a] + i / b.n[a]) / (double)(nblk[a]);
-                            return cfg.xmin[a] + i * (cfg.xmax[a] - cfg.xmin[a]);
-                        };
-                        b.xf0[a] = loc(0);
-                        b.dx0[a] = loc(1) - b.xf0[a];
-                    }
-                    // meshgen.cpp:263-283, 339-353
-                    for (int a = 0; a < 3; ++a) {
-                        double s = 1.;
-                        for (int m = 1; m < 3; ++m) s *= b.dx0[(a + m) % 3];
-                        b.sfc0[a] = s;
-                    }
-                    double v = 1.;
-                    for (int a = 0; a < 3; ++a) v *= b.dx0[a];
-                    b.vol0 = v;
-                    // meshgen.cpp:406-423 (physical-boundary flags)
-                    for (int a = 0; a < 3; ++a) {
-                        const bool per = cfg.bnd[2 * a] == KRO_PER;
-                        b.phy[a][0] = !per && b.reg[a] == 0;
-                        b.phy[a][1] = !per && b.reg[a] == nblk[a] - 1;
-                    }
-                    const int zero[3] = {0, 0, 0}, one[3] = {1, 1, 1};
-                    b.u.init(b.n, zero, nv, 2);
-                    b.w.init(b.n, b.gw, nv, 1);
-                    for (int a = 0; a < b.ndim; ++a) {
-                        int nf[3] = {b.n[0], b.n[1], b.n[2]};
-                        nf[a] += 1;
-                        b.f[a].init(nf, one, nv, 1);
-                    }
-                    if (cfg.mhd) {                  // fld_data.cpp:18-26
-                        int nf[3];
-                        for (int a = 0; a < 3; ++a) nf[a] = b.n[a] + (b.n[a] > 1);
-                        const int g2[3] = {2, 2, 2};
-                        b.bf.init(nf, g2, 3, 2);
-                        b.ec.init(nf, g2, 3, 1);
-                        b.ee.init(nf, one, 3, 1);
-                    }
-                }
+        } else {
+            blocks.resize(levels->size());
+            for (size_t b = 0; b < levels->size(); ++b)
+                init_block(blocks[b], (*levels)[b], &(*regs)[3 * b]);
+            build_smr();
+        }
         // cycle.cpp:163-166: seed of the device dt
         p_dt = cfg.dt_init < FLT_MIN ? (double)(FLT_MAX / 10) : cfg.dt_init;
         t = 0;
         dt_cur = cfg.dt_init;
+    }
+
+    void init_block(Block &b, int level, const int r[3]) {
+        const int nv = nvar();
+        b.ndim = 0;
+        b.level = level;
+        for (int a = 0; a < 3; ++a) {
+            b.reg[a] = r[a];
+            b.n[a] = cfg.nb[a];
+            b.ndim += (b.n[a] > 1 ? 1 : 0);     // geometry.cpp:46
+            b.gw[a] = b.n[a] > 1 ? 2 : 0;       // hyd_data.cpp:21-24
+        }
+        // meshgen.cpp:286-321 (uniform: loc(0), loc(1) - loc(0))
+        for (int a = 0; a < 3; ++a) {
+            auto loc = [&](double i) -> double {
+                if (b.n[a] == 1) return i;
+                i = (b.reg[a] + i / b.n[a]) / (double)(nblk[a] << level);
+                return cfg.xmin[a] + i * (cfg.xmax[a] - cfg.xmin[a]);
+            };
+            b.xf0[a] = loc(0);
+            b.dx0[a] = loc(1) - b.xf0[a];
+        }
+        // meshgen.cpp:263-283, 339-353
+        for (int a = 0; a < 3; ++a) {
+            double s = 1.;
+            for (int m = 1; m < 3; ++m) s *= b.dx0[(a + m) % 3];
+            b.sfc0[a] = s;
+        }
+        double v = 1.;
+        for (int a = 0; a < 3; ++a) v *= b.dx0[a];
+        b.vol0 = v;
+        // meshgen.cpp:406-423 (physical-boundary flags)
+        for (int a = 0; a < 3; ++a) {
+            const bool per = cfg.bnd[2 * a] == KRO_PER;
+            b.phy[a][0] = !per && b.reg[a] == 0;
+            b.phy[a][1] = !per && b.reg[a] == (nblk[a] << level) - 1;
+        }
+        for (int d = 0; d < 27; ++d) b.lev_nb[d] = 0;
+        const int zero[3] = {0, 0, 0}, one[3] = {1, 1, 1};
+        b.u.init(b.n, zero, nv, 2);
+        b.w.init(b.n, b.gw, nv, 1);
+        for (int a = 0; a < b.ndim; ++a) {
+            int nf[3] = {b.n[0], b.n[1], b.n[2]};
+            nf[a] += 1;
+            b.f[a].init(nf, one, nv, 1);
+        }
+        if (cfg.mhd) {                  // fld_data.cpp:18-26
+            int nf[3];
+            for (int a = 0; a < 3; ++a) nf[a] = b.n[a] + (b.n[a] > 1);
+            const int g2[3] = {2, 2, 2};
+            b.bf.init(nf, g2, 3, 2);
+            b.ec.init(nf, g2, 3, 1);
+            b.ee.init(nf, one, 3, 1);
+        }
+    }
+
+    /* -------- static mesh refinement (SMR) ----------------------------
+     * Blocks of one size at several levels, 2:1 balanced (tree.cpp:185-205
+     * insert()).  The leaf covering any position is found through one
+     * dense table per level (the tree_t lookup of tree.cpp:54-58).      */
+    bool smr = false;
+    int nlev = 1;
+    std::vector<std::vector<int>> table;   // [level][logical index] -> block or -1
+
+    size_t lidx(int L, const int r[3]) const {
+        return ((size_t)r[2] * (size_t)(nblk[1] << L) + (size_t)r[1]) * (size_t)(nblk[0] << L)
+               + (size_t)r[0];
+    }
+    int leaf(int L, const int r[3]) const {
+        if (L < 0 || L >= nlev) return -1;
+        for (int a = 0; a < 3; ++a)
+            if (r[a] < 0 || r[a] >= (nblk[a] << L)) return -1;
+        return table[L][lidx(L, r)];
+    }
+    // Wrap a global cell index of level L into the domain along periodic
+    // axes (tree.cpp:77-84); false outside a non-periodic side.
+    bool wrap(int L, int G[3]) const {
+        for (int a = 0; a < 3; ++a) {
+            const int N = (nblk[a] * cfg.nb[a]) << L;
+            if (G[a] >= 0 && G[a] < N) continue;
+            if (cfg.bnd[2 * a] != KRO_PER) return false;
+            G[a] = ((G[a] % N) + N) % N;
+        }
+        return true;
+    }
+    // Leaf covering global cell G of level L (in the domain): the relative
+    // level of that leaf (0, -1 coarser, +1 finer) and its block index.
+    int cover(int L, const int G[3], int &blk) const {
+        int r[3];
+        for (int a = 0; a < 3; ++a) r[a] = G[a] / cfg.nb[a];
+        if ((blk = leaf(L, r)) >= 0) return 0;
+        for (int a = 0; a < 3; ++a) r[a] = (G[a] >> 1) / cfg.nb[a];
+        if ((blk = leaf(L - 1, r)) >= 0) return -1;
+        for (int a = 0; a < 3; ++a) r[a] = (2 * G[a]) / cfg.nb[a];
+        if ((blk = leaf(L + 1, r)) >= 0) return 1;
+        throw std::runtime_error("SMR: a ghost cell has no leaf within one level");
+    }
+
+    void build_smr() {
+        smr = true;
+        if (ndim != 3) throw std::runtime_error("SMR meshes are 3-D");
+        for (int a = 0; a < 3; ++a)
+            if (cfg.nb[a] < 4 || cfg.nb[a] % 2)
+                throw std::runtime_error("SMR blocks need an even size >= 4 per axis");
+        nlev = 0;
+        for (auto &b : blocks) nlev = std::max(nlev, b.level + 1);
+        table.assign(nlev, std::vector<int>());
+        for (int L = 0; L < nlev; ++L)
+            table[L].assign((size_t)(nblk[0] << L) * (nblk[1] << L) * (nblk[2] << L), -1);
+        for (size_t bi = 0; bi < blocks.size(); ++bi) {
+            const Block &b = blocks[bi];
+            if (b.level < 0 || leaf(b.level, b.reg) != -1) {
+                for (int a = 0; a < 3; ++a)
+                    if (b.reg[a] < 0 || b.reg[a] >= (nblk[a] << b.level))
+                        throw std::runtime_error("SMR: block outside the domain");
+                if (b.level < 0 || table[b.level][lidx(b.level, b.reg)] != -1)
+                    throw std::runtime_error("SMR: duplicate block");
+            }
+            table[b.level][lidx(b.level, b.reg)] = (int)bi;
+        }
+        // neighbour levels per direction (tree.cpp:177-197 neighbors())
+        for (auto &b : blocks)
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int d[3] = {dx, dy, dz};
+                        if (!dx && !dy && !dz) continue;
+                        int G[3];
+                        for (int a = 0; a < 3; ++a)
+                            G[a] = (b.reg[a] + d[a]) * b.n[a] + (d[a] < 0 ? b.n[a] - 1 : 0);
+                        int &lv = b.lev_nb[dir_index(d)];
+                        if (!wrap(b.level, G)) { lv = kOut; continue; }
+                        int src;
+                        lv = cover(b.level, G, src);
+                        // the whole neighbour region must be one level
+                        for (int c = 0; c < 8; ++c) {
+                            int H[3];
+                            for (int a = 0; a < 3; ++a)
+                                H[a] = (b.reg[a] + d[a]) * b.n[a] + ((c >> a) & 1) * (b.n[a] - 1);
+                            wrap(b.level, H);
+                            int s2;
+                            if (cover(b.level, H, s2) != lv)
+                                throw std::runtime_error("SMR: mesh is not 2:1 balanced");
+                        }
+                    }
+    }
+    bool cross_face(const Block &b, int a, int s) const {
+        int d[3] = {0, 0, 0};
+        d[a] = s ? 1 : -1;
+        const int lv = b.lev_nb[dir_index(d)];
+        return lv == 1 || lv == -1;
+    }
+    bool refine_face(const Block &b, int a, int s) const {   // geometry refine_flag
+        int d[3] = {0, 0, 0};
+        d[a] = s ? 1 : -1;
+        return smr && b.lev_nb[dir_index(d)] == 1;
+    }
+
+    /* Cross-level ghost cells of w.  Restriction: volume-weighted mean of
+     * the 2x2x2 finer cells (hydro/boundary/comm_functions.h:329-380).
+     * Prolongation: the covering coarser cell plus minmod-limited slopes,
+     * dq = minmod(q_r - q_c, q_c - q_l) / 4 per axis, signed by the fine
+     * cell's half (comm_functions.h:382-452, mhd :470-571).  Both sum in
+     * x, y, z order, so every block sees the same value at one position.
+     * Same-level and restricted ghosts come first (act_bnd's q_ghst
+     * stage), prolongations last (q_plgt, hydro/boundary/keeper.cpp
+     * :166-179), so the coarse slopes read filled ghosts.               */
+    void smr_ghost(int bi, int phase) {
+        Block &b = blocks[bi];
+        const int nv = nvar();
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = -b.gw[a]; hi[a] = b.n[a] + b.gw[a]; }
+        for (int k = lo[2]; k < hi[2]; ++k)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int i = lo[0]; i < hi[0]; ++i) {
+                    const int loc[3] = {i, j, k};
+                    int nout = 0;
+                    for (int a = 0; a < 3; ++a) nout += (loc[a] < 0 || loc[a] >= b.n[a]);
+                    if (nout == 0) continue;
+                    if (!cfg.mhd && nout > 1) continue;
+                    int G[3];
+                    for (int a = 0; a < 3; ++a) G[a] = b.reg[a] * b.n[a] + loc[a];
+                    if (!wrap(b.level, G)) continue;
+                    int src;
+                    const int rel = cover(b.level, G, src);
+                    const Block &s = blocks[src];
+                    double *d = b.w.at(loc);
+                    if (phase == 1 && rel == 0) {
+                        int l[3];
+                        for (int a = 0; a < 3; ++a) l[a] = G[a] - s.reg[a] * s.n[a];
+                        const double *p = s.w.at(l);
+                        for (int v = 0; v < nv; ++v) d[v] = p[v];
+                    } else if (phase == 1 && rel == 1) {
+                        int l[3];
+                        for (int a = 0; a < 3; ++a) l[a] = 2 * G[a] - s.reg[a] * s.n[a];
+                        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, norm = 0;
+                        const double dv = s.vol0;
+                        for (int k0 = 0; k0 < 2; ++k0)
+                            for (int d1 = 0; d1 < 2; ++d1)
+                                for (int d2 = 0; d2 < 2; ++d2) {
+                                    const double *p = s.w.at(l[0] + k0, l[1] + d1, l[2] + d2);
+                                    for (int v = 0; v < nv; ++v) acc[v] += p[v] * dv;
+                                    norm += dv;
+                                }
+                        for (int v = 0; v < nv; ++v) d[v] = acc[v] / norm;
+                    } else if (phase == 2 && rel == -1) {
+                        int l[3];
+                        for (int a = 0; a < 3; ++a) l[a] = (G[a] >> 1) - s.reg[a] * s.n[a];
+                        const double *qc = s.w.at(l);
+                        double dq[3][8];
+                        for (int a = 0; a < 3; ++a) {
+                            int lr[3] = {l[0], l[1], l[2]}, ll[3] = {l[0], l[1], l[2]};
+                            ++lr[a];
+                            --ll[a];
+                            for (const int *q : {lr, ll}) {
+                                int dd[3], out = 0;
+                                for (int x = 0; x < 3; ++x) {
+                                    dd[x] = q[x] < 0 ? -1 : (q[x] >= s.n[x] ? 1 : 0);
+                                    out |= dd[x];
+                                }
+                                if (out && s.lev_nb[dir_index(dd)] == -1)
+                                    throw std::runtime_error("SMR: prolongation stencil "
+                                                             "reaches a coarser ghost");
+                            }
+                            const double *qr = s.w.at(lr), *ql = s.w.at(ll);
+                            for (int v = 0; v < nv; ++v)
+                                dq[a][v] = minmod(qr[v] - qc[v], qc[v] - ql[v]) * 0.25f;
+                        }
+                        int sg[3];
+                        for (int a = 0; a < 3; ++a) sg[a] = 2 * (G[a] & 1) - 1;
+                        for (int v = 0; v < nv; ++v)
+                            d[v] = qc[v] + dq[0][v] * sg[0] + dq[1][v] * sg[1] + dq[2][v] * sg[2];
+                    }
+                }
+    }
+
+    // Tangential face field of same-level face neighbours, one layer
+    // (mhd/boundary/comm_functions.h:433-465); cross-level faces carry none.
+    void smr_bf_ghost(int bi, int step) {
+        Block &b = blocks[bi];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int side = 0; side < 2; ++side) {
+                int d[3] = {0, 0, 0};
+                d[ax] = side ? 1 : -1;
+                if (b.lev_nb[dir_index(d)] != 0) continue;
+                int r[3];
+                for (int a = 0; a < 3; ++a) r[a] = b.reg[a] + d[a];
+                int G[3];
+                for (int a = 0; a < 3; ++a) G[a] = r[a] * b.n[a];
+                wrap(b.level, G);
+                for (int a = 0; a < 3; ++a) r[a] = G[a] / b.n[a];
+                const Block &s = blocks[leaf(b.level, r)];
+                const int a1 = mod3(ax + 1), a2 = mod3(ax + 2);
+                for (int t2 = 0; t2 <= b.n[a2]; ++t2)
+                    for (int t1 = 0; t1 <= b.n[a1]; ++t1) {
+                        int loc[3], sl[3];
+                        loc[ax] = side ? b.n[ax] : -1;
+                        sl[ax] = side ? 0 : b.n[ax] - 1;
+                        loc[a1] = sl[a1] = t1;
+                        loc[a2] = sl[a2] = t2;
+                        const double *sp = s.bf.at(sl, step);
+                        double *dp = b.bf.at(loc, step);
+                        dp[a1] = sp[a1];
+                        dp[a2] = sp[a2];
+                    }
+            }
+    }
+
+    /* Flux correction at faces whose neighbour is finer: the coarse face
+     * flux becomes the area-weighted mean of the four finer face fluxes
+     * (hydro/boundary/comm_functions.h:457-486 flux_to_buf, :489-505
+     * flux_from_buf), summed along axis+1 then axis+2.                  */
+    void smr_flux_restrict(Block &b) {
+        const int nv = nvar();
+        for (int a = 0; a < 3; ++a)
+            for (int sd = 0; sd < 2; ++sd) {
+                if (!refine_face(b, a, sd)) continue;
+                const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                for (int j2 = 0; j2 < b.n[a2]; ++j2)
+                    for (int j1 = 0; j1 < b.n[a1]; ++j1) {
+                        int G[3];               // coarse cell just outside the face
+                        G[a] = b.reg[a] * b.n[a] + (sd ? b.n[a] : -1);
+                        G[a1] = b.reg[a1] * b.n[a1] + j1;
+                        G[a2] = b.reg[a2] * b.n[a2] + j2;
+                        wrap(b.level, G);
+                        int src;
+                        if (cover(b.level, G, src) != 1)
+                            throw std::runtime_error("SMR: refined face without finer leaf");
+                        const Block &f = blocks[src];
+                        int l[3];
+                        for (int x = 0; x < 3; ++x) l[x] = 2 * G[x] - f.reg[x] * f.n[x];
+                        l[a] = sd ? 0 : f.n[a];            // the finer block's face on the interface
+                        double fb[8] = {0, 0, 0, 0, 0, 0, 0, 0}, norm = 0;
+                        const double ds = f.sfc0[a];
+                        for (int d1 = 0; d1 < 2; ++d1)
+                            for (int d2 = 0; d2 < 2; ++d2) {
+                                int p[3] = {l[0], l[1], l[2]};
+                                p[a1] += d1;
+                                p[a2] += d2;
+                                const double *fm = f.f[a].at(p);
+                                for (int v = 0; v < nv; ++v) fb[v] += fm[v] * ds;
+                                norm += ds;
+                            }
+                        int q[3];
+                        q[a] = sd ? b.n[a] : 0;
+                        q[a1] = j1;
+                        q[a2] = j2;
+                        double *fc = b.f[a].at(q);
+                        for (int v = 0; v < nv; ++v) fc[v] = fb[v] / norm;
+                    }
+            }
+    }
+
+    /* intg_ker (hydro/integrator/integrator.h:313-382): the corrected flux
+     * of each refined face is added to the adjacent cell of the target
+     * slot after the regular update skipped it (load_flux_sfc's null
+     * faces, :156-182); z sides first, then y, then x.                  */
+    void smr_intg_faces(Block &b, int step, double dt) {
+        const int nv = nvar();
+        for (int a = b.ndim - 1; a >= 0; --a)
+            for (int sd = 0; sd <= 1; ++sd) {
+                if (!refine_face(b, a, sd)) continue;
+                const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                for (int j2 = 0; j2 < b.n[a2]; ++j2)
+                    for (int j1 = 0; j1 < b.n[a1]; ++j1) {
+                        int i[3];
+                        i[a] = sd * b.n[a];
+                        i[a1] = j1;
+                        i[a2] = j2;
+                        const double *f = b.f[a].at(i);
+                        const double ds = b.sfc0[a];
+                        i[a] -= sd;
+                        double *u = b.u.at(i, 1 - step);
+                        const double dv = b.vol0 * (sd ? -1 : 1);
+                        const double conv = dt * ds / dv;
+                        for (int v = 0; v < nv; ++v) u[v] += conv * f[v];
+                    }
+            }
+    }
+
+    /* Edge EMFs where levels meet (mhd/boundary/comm_functions.h:795-864,
+     * :919-954).  An edge of a coarser block that a finer leaf shares takes
+     * the length-weighted mean of the two finer edges along it; afterwards
+     * the finer blocks' edge pairs on such lines are overwritten with that
+     * mean (the write-back), so both sides see one edge value.          */
+    int edge_levels(const Block &b, int a, const int i[3], int &fine_o1, int &fine_o2) const {
+        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+        int has = 0;
+        fine_o1 = fine_o2 = 0;
+        for (int o2 = -1; o2 <= 0; ++o2)
+            for (int o1 = -1; o1 <= 0; ++o1) {
+                int c[3] = {i[0], i[1], i[2]};
+                c[a1] += o1;
+                c[a2] += o2;
+                int d[3];
+                for (int x = 0; x < 3; ++x) d[x] = c[x] < 0 ? -1 : (c[x] >= b.n[x] ? 1 : 0);
+                const int lv = (d[0] || d[1] || d[2]) ? b.lev_nb[dir_index(d)] : 0;
+                if (lv == 1 && !(has & 1)) {
+                    has |= 1;
+                    fine_o1 = o1;
+                    fine_o2 = o2;
+                }
+                if (lv == -1) has |= 2;
+            }
+        return has;
+    }
+    void smr_emf_coarse(Block &b) {
+        for (int i2 = 0; i2 <= b.n[2]; ++i2)
+            for (int i1 = 0; i1 <= b.n[1]; ++i1)
+                for (int i0 = 0; i0 <= b.n[0]; ++i0) {
+                    const int i[3] = {i0, i1, i2};
+                    bool bnd = false;
+                    for (int x = 0; x < 3; ++x) bnd |= i[x] == 0 || i[x] == b.n[x];
+                    if (!bnd) continue;
+                    for (int a = 0; a < 3; ++a) {
+                        if (i[a] >= b.n[a]) continue;
+                        int o1, o2;
+                        if (!(edge_levels(b, a, i, o1, o2) & 1)) continue;
+                        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                        int G[3];
+                        for (int x = 0; x < 3; ++x) G[x] = b.reg[x] * b.n[x] + i[x];
+                        G[a1] += o1;
+                        G[a2] += o2;
+                        wrap(b.level, G);
+                        int src;
+                        cover(b.level, G, src);
+                        const Block &f = blocks[src];
+                        int e[3];
+                        for (int x = 0; x < 3; ++x) e[x] = 2 * G[x] - f.reg[x] * f.n[x];
+                        e[a1] += o1 < 0 ? 2 : 0;           // the finer edge on the line
+                        e[a2] += o2 < 0 ? 2 : 0;
+                        const double dl = f.dx0[a];
+                        const double el = f.ee.at(e)[a] * dl;
+                        ++e[a];
+                        const double er = f.ee.at(e)[a] * dl;
+                        b.ee.at(i)[a] = (el + er) / (dl + dl);
+                    }
+                }
+    }
+    void smr_emf_writeback(Block &b) {
+        for (int i2 = 0; i2 <= b.n[2]; ++i2)
+            for (int i1 = 0; i1 <= b.n[1]; ++i1)
+                for (int i0 = 0; i0 <= b.n[0]; ++i0) {
+                    const int i[3] = {i0, i1, i2};
+                    bool bnd = false;
+                    for (int x = 0; x < 3; ++x) bnd |= i[x] == 0 || i[x] == b.n[x];
+                    if (!bnd) continue;
+                    for (int a = 0; a < 3; ++a) {
+                        if (i[a] >= b.n[a] || (i[a] & 1)) continue;
+                        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                        if ((i[a1] & 1) || (i[a2] & 1)) continue;   // not on a coarser edge line
+                        int o1, o2;
+                        if (!(edge_levels(b, a, i, o1, o2) & 2)) continue;
+                        int j[3] = {i[0], i[1], i[2]};
+                        ++j[a];
+                        const double dl = b.dx0[a];
+                        double *pl = b.ee.at(i) + a, *pr = b.ee.at(j) + a;
+                        const double avg = (*pl * dl + *pr * dl) / (dl + dl);
+                        *pl = avg;
+                        *pr = avg;
+                    }
+                }
     }
 
     /* -------- c2p + CFL (hydro/eos/adiabatic.h:211-259,
@@ -753,10 +1143,17 @@ struct Mesh {
                 for (int v = 0; v < nv; ++v)
                     if (!(mhd && v == BCC + axis)) Q(i)[v] = src[v];
             }
+            // mhd/integrator/data.h:108-140: on a transverse ghost line
+            // next to a cross-level face the normal B is the cell value of w
+            bool cross = false;
+            if (mhd && smr && !interior) {
+                if (t1 < 0 || t1 >= b.n[a1]) cross = cross_face(b, a1, t1 >= 0);
+                else cross = cross_face(b, a2, t2 >= 0);
+            }
             if (mhd)
                 for (int i = 0; i <= n; ++i) {
                     i3[axis] = i;
-                    Q(i)[BCC + axis] = b.bf.at(i3, step)[axis];
+                    Q(i)[BCC + axis] = cross ? b.w.at(i3)[BCC + axis] : b.bf.at(i3, step)[axis];
                 }
             double ql[8], qr[8];
             for (int i = 0; i <= n; ++i) {
@@ -771,7 +1168,7 @@ struct Mesh {
                 }
                 face_flux(mhd, axis, ql, qr, &flx[(size_t)i * nv], eos);
             }
-            if (mhd || axis != 0)                       // save_flux
+            if (mhd || axis != 0 || smr)                // save_flux / slice_flux
                 for (int i = 0; i <= n; ++i) {
                     i3[axis] = i;
                     double *dst = b.f[axis].at(i3);
@@ -791,8 +1188,12 @@ struct Mesh {
                     for (int v = 0; v < nv; ++v) uu[v] = (u_src[v] + u_tgt[v]) * 0.5f;
                 const double dt_dv = dt / b.vol0;
                 for (int ax = 0; ax < b.ndim; ++ax) {
-                    const double fa0 = b.sfc0[ax] * dt_dv;
-                    const double fa1 = b.sfc0[ax] * dt_dv;
+                    // refined faces are skipped here and added by
+                    // smr_intg_faces (integrator.h:156-182 null_l / null_r)
+                    const bool null_l = refine_face(b, ax, 0) && i3[ax] == 0;
+                    const bool null_r = refine_face(b, ax, 1) && i3[ax] + 1 == b.n[ax];
+                    const double fa0 = null_l ? -1 : b.sfc0[ax] * dt_dv;
+                    const double fa1 = null_r ? -1 : b.sfc0[ax] * dt_dv;
                     const double *F0, *F1;
                     if (ax == 0) {
                         F0 = &flx[(size_t)i * nv];
@@ -937,16 +1338,33 @@ struct Mesh {
             p_dt = dt_use * cfg.dt_expand;
             dt_reduced = true;
         }
-        par_blocks([&](int bi) {
-            exchange_block(bi, step);
-        });
-        if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
+        if (!smr) {
+            par_blocks([&](int bi) {
+                exchange_block(bi, step);
+            });
+            if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
+        } else {
+            par_blocks([&](int bi) {
+                smr_ghost(bi, 1);
+                if (cfg.mhd) smr_bf_ghost(bi, step);
+            });
+            if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
+            par_blocks([&](int bi) { smr_ghost(bi, 2); });
+        }
         const double dt = dt_cur * (step == 0 ? 1.0 : 0.5);
         par_blocks([&](int bi) {
             Block &b = blocks[bi];
             for (int ax = b.ndim - 1; ax >= 0; --ax) sweep_block(b, ax, step, dt);
             if (cfg.mhd) calc_field_block(b);
         });
+        if (smr) {                               // act_flx, then intg_flx
+            par_blocks([&](int bi) { smr_flux_restrict(blocks[bi]); });
+            if (cfg.mhd) {
+                par_blocks([&](int bi) { smr_emf_coarse(blocks[bi]); });
+                par_blocks([&](int bi) { smr_emf_writeback(blocks[bi]); });
+            }
+            par_blocks([&](int bi) { smr_intg_faces(blocks[bi], step, dt); });
+        }
         if (cfg.mhd)
             par_blocks([&](int bi) { intg_field_block(blocks[bi], step, dt); });
     }
@@ -970,7 +1388,7 @@ extern "C" {
 
 const char *kro_last_error(void) { return g_err.c_str(); }
 
-kro_mesh *kro_create(const kro_config *c) {
+static kro_mesh *kro_create_impl(const kro_config *c, bool do_build) {
     try {
         auto *km = new kro_mesh();
         Mesh &m = km->m;
@@ -1003,13 +1421,35 @@ kro_mesh *kro_create(const kro_config *c) {
         m.eos.gamma = c->gamma; m.eos.cfl = c->cfl;
         m.eos.d_floor = c->rho_floor; m.eos.t_floor = c->tmp_floor;
         m.eos.nhyd = c->mhd ? 8 : 5;
-        m.build();
+        if (do_build) m.build();
         return km;
     } catch (const std::exception &e) {
         g_err = e.what();
         return nullptr;
     }
 }
+
+kro_mesh *kro_create_blocks(const kro_config *c, int n, const int *levels, const int *regs) {
+    try {
+        if (n <= 0 || !levels || !regs) throw std::runtime_error("no blocks");
+        kro_config c0 = *c;
+        auto *km = kro_create_impl(&c0, false);
+        if (!km) return nullptr;
+        std::vector<int> lv(levels, levels + n), rg(regs, regs + 3 * (size_t)n);
+        try {
+            km->m.build(&lv, &rg);
+        } catch (...) {
+            delete km;
+            throw;
+        }
+        return km;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+kro_mesh *kro_create(const kro_config *c) { return kro_create_impl(c, true); }
 
 void kro_destroy(kro_mesh *m) { delete m; }
 

@@ -38,11 +38,18 @@ typedef struct kro_config {
 typedef struct kro_mesh kro_mesh;
 
 kro_mesh *kro_create(const kro_config *cfg);
+/* A statically refined mesh: n blocks of n_cell_block cells, block b at
+ * refinement level levels[b] with logical index regs[3b..3b+2] of that
+ * level (meshgen.cpp:111-158 load_refinement / generate_tree); the root
+ * grid is n_cell_global / n_cell_block blocks.  3-D, 2:1 balanced.     */
+kro_mesh *kro_create_blocks(const kro_config *cfg, int n, const int *levels,
+                            const int *regs);
 void      kro_destroy(kro_mesh *m);
 const char *kro_last_error(void);
 
 int  kro_n_blocks(const kro_mesh *m);
-/* reg: logical block index, n: cells, xf0/dx0: per-block geometry */
+/* reg: logical block index (of the block's level), n: cells, xf0/dx0:
+ * per-block geometry */
 void kro_block_geom(const kro_mesh *m, int b, int reg[3], int n[3],
                     double xf0[3], double dx0[3]);
 int  kro_set_block(kro_mesh *m, int b, const double *u_aos,
