@@ -18,6 +18,10 @@ t = 0.1 (t_lim top-up); relative L1 of rho, E and B of the mixed form
 against the FP64 form (which equals the CPU oracle bit for bit on every
 case the parity tests run), and max |div B|.
 
+smr_blast: the MHD blast on a three-level statically refined mesh (128^3
+root cells in 16^3 blocks, ~2,000 blocks), both forms on the staged kernels
+with the cross-level corrections; cell-updates/s counts every block's cells.
+
 ensemble: 64 independent 64^3 periodic boxes (seeds 0..63, beta drawn from
 U[0.1, 10]) in one context, one time group per box (per-box dt), all boxes
 advanced by the same launches to t = 0.05; aggregate cell-updates/s of the
@@ -161,6 +165,59 @@ def ensemble_leg(n_box, n, t_lim, path):
             "divb_max_mixed": res["mixed"]["divb"]}
 
 
+def smr_leg(n_root, nb, steps, warmup, cycles_acc):
+    """MHD blast on a statically refined mesh: [-0.5, 0.5]^3 periodic,
+    n_root^3 root cells in nb^3 blocks, level 1 on |x_a| <= 0.25 and level 2
+    on |x_a| <= 0.125 (tree-balanced), all blocks one size; both forms on the
+    staged kernels with the cross-level halo, flux and EMF corrections.
+    Reports each form's cell-updates/s over `steps` replayed cycles, and the
+    mixed form's density L1 against FP64 after `cycles_acc` cycles."""
+    import kratos_amd
+    from kratos_amd import problems
+    spec = kratos_amd.RefinedMeshSpec((n_root,) * 3, (nb,) * 3,
+                                      refine=[(1, (-0.25,) * 3, (0.25,) * 3),
+                                              (2, (-0.125,) * 3, (0.125,) * 3)],
+                                      x_min=(-0.5,) * 3, x_max=(0.5,) * 3)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    blocks = problems.blast_blocks(spec)
+    out = {"workload": f"MHD blast, {n_root}^3 root cells, {spec.n_blocks} blocks of {nb}^3 on "
+                       f"{spec.n_levels} levels", "blocks": spec.n_blocks,
+           "blocks_per_level": [sum(1 for lv in spec.leaves if lv[0] == L)
+                                for L in range(spec.n_levels)],
+           "cells": spec.n_cells}
+    states = {}
+    for form in ("fp64", "mixed"):
+        ctx = kratos_amd.Context(spec, phys, form)
+        ctx.upload_blocks(blocks)
+        ctx.advance(cycles_acc)
+        sync_all(ctx)
+        states[form] = ctx.download_blocks()
+        out[f"divb_max_{form}"] = ctx.divb_max()
+        ctx.advance(warmup)
+        sync_all(ctx)
+        t0 = time.perf_counter()
+        done = ctx.advance(steps)
+        sync_all(ctx)
+        wall = time.perf_counter() - t0
+        out[f"cell_updates_per_s_{form}"] = spec.n_cells * done / wall
+        out[f"ms_per_cycle_{form}"] = wall / max(done, 1) * 1e3
+        prof = ctx.profile(2)
+        out[f"kernel_ms_{form}"] = {k: round(v, 4) for k, v in prof["kernel_ms"].items()}
+        ctx.close()
+    num = den = 0.0
+    for b in range(spec.n_blocks):
+        _, dx = spec.geometry(b)
+        v = dx[0] * dx[1] * dx[2]
+        m = np.asarray(states["mixed"][b][0]).reshape(-1, 8)[:, 0]
+        d = np.asarray(states["fp64"][b][0]).reshape(-1, 8)[:, 0]
+        num += np.abs(m - d).sum() * v
+        den += np.abs(d).sum() * v
+    out["cycles_acc"] = cycles_acc
+    out["l1_rho_mixed_vs_fp64"] = num / den
+    out["speedup_mixed_over_fp64"] = out["cell_updates_per_s_mixed"] / out["cell_updates_per_s_fp64"]
+    return out
+
+
 def cpu_baseline(seconds_target=15.0, threads=None):
     """Oracle on a bounded sample: 64^3 cells of the same recipe (8 blocks of 32^3)."""
     import kratos_amd
@@ -228,6 +285,9 @@ def main():
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-n", type=int, default=64)
     ap.add_argument("--t-ens", type=float, default=0.05)
+    ap.add_argument("--no-smr", action="store_true")
+    ap.add_argument("--smr-root", type=int, default=128)
+    ap.add_argument("--smr-block", type=int, default=16)
     ap.add_argument("--tile", default="",
                     help="force the fused-kernel variant as TX,TY,WAVES (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -311,6 +371,10 @@ def main():
     if not args.no_ensemble:
         ens = ensemble_leg(args.ens_boxes, args.ens_n, args.t_ens, args.path)
         print(f"[bench] ensemble: {ens}", file=sys.stderr, flush=True)
+    smr = None
+    if not args.no_smr:
+        smr = smr_leg(args.smr_root, args.smr_block, max(args.steps // 2, 4), 2, 20)
+        print(f"[bench] smr: {smr}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     line = {
         "metric": "MHD cell-updates/s (mixed vs FP64 at stated L1 error), % of HBM roofline",
@@ -338,6 +402,7 @@ def main():
                                     if "mixed" in res and "fp64" in res else None),
         "accuracy_mixed_vs_fp64": acc,
         "ensemble": ens,
+        "smr_blast": smr,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

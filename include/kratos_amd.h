@@ -57,12 +57,24 @@ enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };
 
 typedef struct kr_block_desc {
     int    id;             /* caller's block id                              */
-    int    level;          /* refinement level (same-level meshes only)      */
+    int    level;          /* refinement level.  When any block has level > 0
+                              the mesh is statically refined: neighbours are
+                              derived from level + reg as tree_t::neighbors
+                              does (mesh/tree/tree.cpp:177-197), nbr[] is not
+                              read, and the mesh must be 3-D, periodic or
+                              physical per axis, one time group, 2:1 balanced,
+                              blocks of one even size >= 4 per axis.  Where
+                              levels meet, the library fills the ghost
+                              primitives by restriction / minmod prolongation,
+                              corrects the coarser face fluxes and the shared
+                              edge EMFs (hydro/boundary/comm_functions.h
+                              :329-544, mhd/boundary/comm_functions.h
+                              :467-954, hydro/integrator/integrator.h:313-382) */
     int    group;          /* time group: one mesh with its own t and dt.
                               0 for a single mesh; an ensemble of
                               independent boxes uses 0 .. n_box-1, and blocks
                               of different groups never neighbour.         */
-    int    reg[3];         /* logical block index                            */
+    int    reg[3];         /* logical block index at the block's level       */
     int    n_ceff[3];      /* cells per axis (== kr_config.n_cell_block)     */
     double xf0[3], dx0[3]; /* geometry, as meshgen computes it               */
     int    nbr[27];        /* neighbour block id per direction

@@ -3,6 +3,7 @@
 // the hipGraph replay of whole cycles.
 #include "../../include/kratos_amd.h"
 #include "kr_fused.h"
+#include "kr_smr.h"
 
 #include <hip/hip_runtime.h>
 
@@ -137,6 +138,17 @@ struct kr_ctx {
     int ppb = 0;                    // CFL partials per block
     double *dt_part = nullptr, *red_part = nullptr, *red_out = nullptr;
     size_t n_dt_part = 0, n_red_part = 0;
+    // refined mesh (blocks at several levels)
+    bool smr = false;
+    int nhalo = 0;                  // halo blocks after the real ones in W
+    SmrView sv{};
+    int *d_table = nullptr;
+    int4 *d_fill[2] = {nullptr, nullptr};   // restriction, prolongation tasks
+    int nfill[2] = {0, 0};
+    int2 *d_ftask = nullptr;        // faces whose neighbour is finer
+    int nftask = 0;
+    int *d_xblk = nullptr, *d_rblk = nullptr;   // blocks with cross-level / refined sides
+    int nxblk = 0, nrblk = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -176,9 +188,14 @@ void release(kr_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
     void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE,
-                    c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out};
+                    c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
+                    c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    c->d_table = nullptr;
+    c->d_fill[0] = c->d_fill[1] = nullptr;
+    c->d_ftask = nullptr;
+    c->d_xblk = c->d_rblk = nullptr;
     c->U = c->BF = nullptr;
     c->W = c->EE = nullptr;
     c->F[0] = c->F[1] = c->F[2] = nullptr;
@@ -193,6 +210,205 @@ template <class T> T *dalloc(size_t count) {
     if (count == 0) count = 1;
     KR_HIP(hipMalloc(&p, count * sizeof(T)));
     return (T *)p;
+}
+
+// Topology of a refined mesh, derived from each block's level and logical
+// index as tree_t::neighbors does (mesh/tree/tree.cpp:177-197): for every
+// direction the leaf covering the neighbour region is at the same level
+// (neighbour index), one level coarser or finer (a halo block and a fill
+// task), or outside a non-periodic side (-1).  Validates the 2:1 balance
+// and that no prolongation reads a halo that is itself prolonged.
+struct SmrBuild {
+    int R[3], nlev = 0, n[3];
+    bool per[3];
+    std::vector<std::vector<int>> table;
+    size_t lidx(int L, const int *r) const {
+        return ((size_t)r[2] * (size_t)(R[1] << L) + (size_t)r[1]) * (size_t)(R[0] << L) + (size_t)r[0];
+    }
+    int leaf(int L, const int *r) const {
+        if (L < 0 || L >= nlev) return -1;
+        for (int a = 0; a < 3; ++a)
+            if (r[a] < 0 || r[a] >= (R[a] << L)) return -1;
+        return table[L][lidx(L, r)];
+    }
+    bool wrap(int L, int *G) const {
+        for (int a = 0; a < 3; ++a) {
+            const int N = (R[a] * n[a]) << L;
+            if (G[a] >= 0 && G[a] < N) continue;
+            if (!per[a]) return false;
+            G[a] = ((G[a] % N) + N) % N;
+        }
+        return true;
+    }
+    int cover(int L, const int *G, int &blk) const {
+        int r[3];
+        for (int a = 0; a < 3; ++a) r[a] = G[a] / n[a];
+        if ((blk = leaf(L, r)) >= 0) return 0;
+        for (int a = 0; a < 3; ++a) r[a] = (G[a] >> 1) / n[a];
+        if ((blk = leaf(L - 1, r)) >= 0) return -1;
+        for (int a = 0; a < 3; ++a) r[a] = (2 * G[a]) / n[a];
+        if ((blk = leaf(L + 1, r)) >= 0) return 1;
+        throw std::invalid_argument("refined mesh: a neighbour region has no leaf within one level");
+    }
+};
+
+void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
+    const size_t nb = c->descs.size();
+    if (c->ngrp != 1) throw std::invalid_argument("refined meshes use one time group");
+    if (c->ndim != 3) throw std::invalid_argument("refined meshes are 3-D");
+    if (c->cfg.kernel_path == KR_PATH_FUSED)
+        throw std::invalid_argument("refined meshes run the staged kernels");
+    SmrBuild S;
+    for (int a = 0; a < 3; ++a) {
+        S.n[a] = c->n[a];
+        S.per[a] = c->cfg.bnd[2 * a] == KR_PER;
+        if (c->n[a] < 4 || (c->n[a] & 1))
+            throw std::invalid_argument("refined meshes need an even block size >= 4");
+        S.R[a] = 0;
+    }
+    for (auto &d : c->descs) {
+        if (d.level < 0 || d.level > 7) throw std::invalid_argument("refinement level out of 0..7");
+        S.nlev = std::max(S.nlev, d.level + 1);
+        for (int a = 0; a < 3; ++a)
+            S.R[a] = std::max(S.R[a], (d.reg[a] + 1 + (1 << d.level) - 1) >> d.level);
+    }
+    S.table.assign(S.nlev, {});
+    for (int L = 0; L < S.nlev; ++L)
+        S.table[L].assign((size_t)(S.R[0] << L) * (S.R[1] << L) * (S.R[2] << L), -1);
+    for (size_t b = 0; b < nb; ++b) {
+        const kr_block_desc &d = c->descs[b];
+        for (int a = 0; a < 3; ++a)
+            if (d.reg[a] < 0) throw std::invalid_argument("negative logical block index");
+        int &slot = S.table[d.level][S.lidx(d.level, d.reg)];
+        if (slot >= 0) throw std::invalid_argument("two blocks at one position");
+        slot = (int)b;
+    }
+    std::vector<int4> fill[2];
+    std::vector<int2> ftask;
+    std::vector<int> xblk, rblk;
+    int nhalo = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        const kr_block_desc &d = c->descs[b];
+        BlockGeo &G = g[b];
+        G.level = d.level;
+        for (int a = 0; a < 3; ++a) G.reg[a] = d.reg[a];
+        G.rface = G.xface = 0;
+        for (int k = 0; k < 27; ++k) {
+            G.lev[k] = 0;
+            G.nbr[k] = k == 13 ? (int)b : -1;
+            if (k == 13) continue;
+            const int dd[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+            int lv = 9, src = -1;
+            for (int cnr = 0; cnr < 8; ++cnr) {     // the ghost cells must lie on one level
+                int H[3];
+                for (int a = 0; a < 3; ++a) {
+                    const int o = (cnr >> a) & 1;
+                    H[a] = d.reg[a] * S.n[a] + (dd[a] < 0 ? -2 + o : dd[a] > 0 ? S.n[a] + o
+                                                                               : o * (S.n[a] - 1));
+                }
+                if (!S.wrap(d.level, H)) { lv = 9; break; }
+                int s2;
+                const int l2 = S.cover(d.level, H, s2);
+                if (cnr == 0) { lv = l2; src = s2; }
+                else if (l2 != lv) throw std::invalid_argument("refined mesh is not 2:1 balanced");
+            }
+            G.lev[k] = (signed char)lv;
+            if (lv == 0) G.nbr[k] = src;
+            else if (lv == 1 || lv == -1) {
+                const int hb = (int)nb + nhalo++;
+                G.nbr[k] = hb;
+                fill[lv == 1 ? 0 : 1].push_back(make_int4(hb, (int)b, k, 0));
+            }
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int sd = 0; sd < 2; ++sd) {
+                int dir[3] = {0, 0, 0};
+                dir[a] = sd ? 1 : -1;
+                const int lv = G.lev[((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1)];
+                if (lv == 1) {
+                    G.rface |= 1 << (2 * a + sd);
+                    ftask.push_back(make_int2((int)b, 2 * a + sd));
+                }
+                if (lv == 1 || lv == -1) G.xface |= 1 << (2 * a + sd);
+            }
+        bool any = false, fin = false;
+        for (int k = 0; k < 27; ++k) {
+            any |= G.lev[k] == 1 || G.lev[k] == -1;
+            fin |= G.lev[k] == 1;
+        }
+        if (any) xblk.push_back((int)b);
+        if (G.rface) rblk.push_back((int)b);
+        (void)fin;
+    }
+    // a prolongation's slopes read the coarser block's cells one step past
+    // the covering cell; those must never be prolonged themselves
+    for (const int4 &t : fill[1]) {
+        const BlockGeo &G = g[t.y];
+        const int dd[3] = {t.z % 3 - 1, (t.z / 3) % 3 - 1, t.z / 9 - 1};
+        int ext[3], lo[3];
+        for (int a = 0; a < 3; ++a) {
+            ext[a] = dd[a] == 0 ? S.n[a] : 2;
+            lo[a] = dd[a] < 0 ? -2 : (dd[a] > 0 ? S.n[a] : 0);
+        }
+        for (int z = 0; z < ext[2]; ++z)
+            for (int y = 0; y < ext[1]; ++y)
+                for (int x = 0; x < ext[0]; ++x) {
+                    int Gc[3] = {G.reg[0] * S.n[0] + lo[0] + x, G.reg[1] * S.n[1] + lo[1] + y,
+                                 G.reg[2] * S.n[2] + lo[2] + z};
+                    S.wrap(G.level, Gc);
+                    int cb;
+                    S.cover(G.level, Gc, cb);
+                    const BlockGeo &C = g[cb];
+                    int l[3];
+                    for (int a = 0; a < 3; ++a) l[a] = (Gc[a] >> 1) - C.reg[a] * S.n[a];
+                    for (int a = 0; a < 3; ++a)
+                        for (int sgn = -1; sgn <= 1; sgn += 2) {
+                            int q[3] = {l[0], l[1], l[2]};
+                            q[a] += sgn;
+                            int e[3];
+                            for (int x2 = 0; x2 < 3; ++x2) e[x2] = q[x2] < 0 ? -1 : (q[x2] >= S.n[x2] ? 1 : 0);
+                            const int k = ((e[2] + 1) * 3 + (e[1] + 1)) * 3 + (e[0] + 1);
+                            if (k != 13 && C.lev[k] == -1)
+                                throw std::invalid_argument("refined mesh: a prolongation needs a "
+                                                            "prolonged halo (refine more widely)");
+                        }
+                }
+    }
+    c->smr = true;
+    c->nhalo = nhalo;
+    c->fused = false;
+    for (int a = 0; a < 3; ++a) {
+        c->sv.R[a] = S.R[a];
+        c->sv.per[a] = S.per[a] ? 1 : 0;
+    }
+    c->sv.nlev = S.nlev;
+    std::vector<int> flat;
+    for (int L = 0; L < S.nlev; ++L) {
+        c->sv.toff[L] = (long long)flat.size();
+        flat.insert(flat.end(), S.table[L].begin(), S.table[L].end());
+    }
+    c->d_table = dalloc<int>(flat.size());
+    KR_HIP(hipMemcpy(c->d_table, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->sv.table = c->d_table;
+    for (int k = 0; k < 2; ++k) {
+        c->nfill[k] = (int)fill[k].size();
+        c->d_fill[k] = dalloc<int4>(fill[k].size());
+        if (!fill[k].empty())
+            KR_HIP(hipMemcpy(c->d_fill[k], fill[k].data(), fill[k].size() * sizeof(int4),
+                             hipMemcpyHostToDevice));
+    }
+    c->nftask = (int)ftask.size();
+    c->d_ftask = dalloc<int2>(ftask.size());
+    if (!ftask.empty())
+        KR_HIP(hipMemcpy(c->d_ftask, ftask.data(), ftask.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c->nxblk = (int)xblk.size();
+    c->d_xblk = dalloc<int>(xblk.size());
+    if (!xblk.empty())
+        KR_HIP(hipMemcpy(c->d_xblk, xblk.data(), xblk.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->nrblk = (int)rblk.size();
+    c->d_rblk = dalloc<int>(rblk.size());
+    if (!rblk.empty())
+        KR_HIP(hipMemcpy(c->d_rblk, rblk.data(), rblk.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
 // Lay out the pools once every block is known (the reference allocates per
@@ -224,16 +440,9 @@ void commit(kr_ctx *c) {
         for (int a = 0; a < 3; ++a) s *= (size_t)(c->n[a] + (a == ax ? 1 : 2));
         c->nfa[ax] = s;
     }
-    const size_t fs = c->mixed ? sizeof(float) : sizeof(double);
-    c->U = dalloc<double>(nb * 2 * 5 * c->ncell);
-    if (c->mhd) c->BF = dalloc<double>(nb * 2 * 3 * c->nface);
-    KR_HIP(hipMalloc(&c->W, nb * c->nv * c->ncell * fs));
-    for (int a = 0; a < c->ndim; ++a) KR_HIP(hipMalloc(&c->F[a], nb * c->nv * c->nfa[a] * fs));
-    if (c->mhd) KR_HIP(hipMalloc(&c->EE, nb * 3 * c->nface * fs));
-    KR_HIP(hipMemset(c->U, 0, nb * 2 * 5 * c->ncell * sizeof(double)));
-    if (c->mhd) KR_HIP(hipMemset(c->BF, 0, nb * 2 * 3 * c->nface * sizeof(double)));
-    KR_HIP(hipMemset(c->W, 0, nb * c->nv * c->ncell * fs));
 
+    bool multi_level = false;
+    for (auto &d : c->descs) multi_level |= d.level != 0;
     std::vector<BlockGeo> g(nb);
     for (size_t b = 0; b < nb; ++b) {
         const kr_block_desc &d = c->descs[b];
@@ -252,7 +461,7 @@ void commit(kr_ctx *c) {
         G.phy = 0;
         G.grp = d.group;
         for (int k = 0; k < 27; ++k) {
-            const int id = d.nbr[k];
+            const int id = multi_level ? -1 : d.nbr[k];   // refined: derived by build_smr
             if (id < 0) {
                 G.nbr[k] = -1;
                 continue;
@@ -272,9 +481,36 @@ void commit(kr_ctx *c) {
                 const int k = ((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1);
                 if (G.nbr[k] < 0) G.phy |= 1 << (2 * a + s);
             }
-        if (c->mhd && G.phy)
+        if (c->mhd && G.phy && !multi_level)
             throw std::invalid_argument("MHD supports periodic / internal sides only");
     }
+    if (multi_level) {
+        build_smr(c, g);
+        for (size_t b = 0; b < nb; ++b) {
+            BlockGeo &G = g[b];
+            G.phy = 0;
+            for (int a = 0; a < 3; ++a)
+                for (int sd = 0; sd < 2; ++sd) {
+                    int dir[3] = {0, 0, 0};
+                    dir[a] = sd ? 1 : -1;
+                    if (G.lev[((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1)] == 9)
+                        G.phy |= 1 << (2 * a + sd);
+                }
+            if (c->mhd && G.phy)
+                throw std::invalid_argument("MHD supports periodic / internal sides only");
+        }
+    }
+    // pools: W also holds the halo blocks of a refined mesh
+    const size_t fs = c->mixed ? sizeof(float) : sizeof(double);
+    const size_t nw = nb + (size_t)c->nhalo;
+    c->U = dalloc<double>(nb * 2 * 5 * c->ncell);
+    if (c->mhd) c->BF = dalloc<double>(nb * 2 * 3 * c->nface);
+    KR_HIP(hipMalloc(&c->W, nw * c->nv * c->ncell * fs));
+    for (int a = 0; a < c->ndim; ++a) KR_HIP(hipMalloc(&c->F[a], nb * c->nv * c->nfa[a] * fs));
+    if (c->mhd) KR_HIP(hipMalloc(&c->EE, nb * 3 * c->nface * fs));
+    KR_HIP(hipMemset(c->U, 0, nb * 2 * 5 * c->ncell * sizeof(double)));
+    if (c->mhd) KR_HIP(hipMemset(c->BF, 0, nb * 2 * 3 * c->nface * sizeof(double)));
+    KR_HIP(hipMemset(c->W, 0, nw * c->nv * c->ncell * fs));
     c->geo = dalloc<BlockGeo>(nb);
     KR_HIP(hipMemcpy(c->geo, g.data(), nb * sizeof(BlockGeo), hipMemcpyHostToDevice));
 
@@ -300,7 +536,7 @@ void commit(kr_ctx *c) {
     // fused stage kernel: 3-D MHD blocks divisible by a supported tile
     // (kr_config.tile forces a shape); the variant is picked by timing at
     // the first step
-    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
+    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED && !c->smr) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {
@@ -387,6 +623,23 @@ struct Launcher {
                 return;
             }
         }
+        constexpr int NV = MHD ? 8 : 5;
+        if (c->smr) {                       // halo where levels meet (accounted as c2p)
+            size_t mx = 0;                  // largest face region: 2 x n x n
+            for (int a = 0; a < 3; ++a) {
+                size_t e = 2;
+                for (int x = 0; x < 3; ++x)
+                    if (x != a) e *= (size_t)c->n[x];
+                mx = std::max(mx, e);
+            }
+            for (int k = 0; k < 2; ++k)
+                if (c->nfill[k])
+                    run(KR_K_C2P, [&] {
+                        hipLaunchKernelGGL((k_smr_fill<FT, NV>), dim3(ceil_div(mx, kSmrThreads), c->nfill[k]),
+                                           dim3(kSmrThreads), 0, c->stream, m, c->sv,
+                                           (const int4 *)c->d_fill[k], k);
+                    });
+        }
         if (c->ndim > 2)
             run(KR_K_FLUX_Z, [&] {
                 hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), nb),
@@ -406,11 +659,37 @@ struct Launcher {
                 hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), nb), blk, 0,
                                    c->stream, m);
             });
+        if (c->smr) {                       // act_flx: EMF and flux restriction
+            if (MHD && c->nxblk) {
+                run(KR_K_EMF, [&] {
+                    hipLaunchKernelGGL(k_smr_emf_coarse<FT>, dim3(ceil_div(c->nface, kSmrThreads), c->nxblk),
+                                       dim3(kSmrThreads), 0, c->stream, m, c->sv, (const int *)c->d_xblk);
+                });
+                run(KR_K_EMF, [&] {
+                    hipLaunchKernelGGL(k_smr_emf_wb<FT>, dim3(ceil_div(c->nface, kSmrThreads), c->nxblk),
+                                       dim3(kSmrThreads), 0, c->stream, m, (const int *)c->d_xblk);
+                });
+            }
+            if (c->nftask)
+                run(KR_K_UPDATE, [&] {
+                    hipLaunchKernelGGL((k_smr_flux<FT, NV>),
+                                       dim3(ceil_div((size_t)c->n[0] * c->n[1] * c->n[2] /
+                                                         std::min(c->n[0], std::min(c->n[1], c->n[2])),
+                                                     kSmrThreads),
+                                            c->nftask),
+                                       dim3(kSmrThreads), 0, c->stream, m, c->sv, (const int2 *)c->d_ftask);
+                });
+        }
         run(KR_K_UPDATE, [&] {
             hipLaunchKernelGGL((k_update<FT, MHD>),
                                dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), nb), blk, 0,
                                c->stream, m, step);
         });
+        if (c->smr && c->nrblk)             // intg_flx: the refined faces
+            run(KR_K_UPDATE, [&] {
+                hipLaunchKernelGGL(k_smr_intg<FT>, dim3(ceil_div(c->ncell, kSmrThreads), c->nrblk),
+                                   dim3(kSmrThreads), 0, c->stream, m, (const int *)c->d_rblk, step);
+            });
     }
 
     template <class FT, bool MHD> void cycle_t() {
@@ -578,8 +857,8 @@ int kr_add_blocks(kr_ctx *c, int n, const kr_block_desc *blocks) {
             for (int a = 0; a < 3; ++a)
                 if (d.n_ceff[a] != c->cfg.n_cell_block[a])
                     throw std::invalid_argument("block size differs from n_cell_block");
-            if (d.level != 0 && d.level != blocks[0].level)
-                throw std::invalid_argument("only same-level meshes are supported");
+            if (d.level < 0 || d.level > 7)
+                throw std::invalid_argument("refinement level out of 0..7");
             if (d.group < 0 || d.group >= (1 << 20)) throw std::invalid_argument("bad time group");
             if (c->id2idx.count(d.id)) throw std::invalid_argument("duplicate block id");
             c->id2idx[d.id] = (int)c->descs.size();

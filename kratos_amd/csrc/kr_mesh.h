@@ -16,10 +16,40 @@ struct BlockGeo {
     double dx[3];      // dx0
     double sfc[3];     // sfc0 (face areas, meshgen.cpp:263-272 product order)
     double vol;        // vol0
-    int nbr[27];       // block index at ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical side
+    int nbr[27];       // block index at ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical side.
+                       // On a refined mesh a direction whose neighbour region lies at
+                       // another level points at a halo block (see SmrView).
     int phy;           // bit (2*axis + side) set on a physical side
     int grp;           // time group: index of this block's TimeState
+    // refined meshes (all zero on a same-level mesh)
+    int level;         // refinement level
+    int reg[3];        // logical index of the block at its level
+    int rface;         // bit (2*axis + side): the face neighbour is finer (refine_flag)
+    int xface;         // bit (2*axis + side): the face neighbour is at another level
+    signed char lev[27];  // level of the leaf covering each direction, relative
+                          // (-1 coarser, 0 same, +1 finer; 9 outside the domain)
 };
+
+// Halo fill of a refined mesh.  Every direction of a block whose neighbour
+// region lies at another level gets a halo block: a block-sized slot of W
+// (index >= the number of real blocks) whose cells at the same local
+// positions a same-level neighbour would have hold the restricted (finer
+// source) or prolonged (coarser source) primitives.  Tasks are {halo
+// block, owner block, direction, 0}.
+struct SmrView {
+    int R[3];          // root blocks per axis
+    int nlev;
+    const int *table;  // leaf block per (level, logical index), -1 if none
+    long long toff[8]; // offset of each level's table
+    int per[3];        // periodic axes
+};
+
+__host__ __device__ inline int smr_leaf(const SmrView &s, int L, const int *r) {
+    if (L < 0 || L >= s.nlev) return -1;
+    for (int a = 0; a < 3; ++a)
+        if (r[a] < 0 || r[a] >= (s.R[a] << L)) return -1;
+    return s.table[s.toff[L] + ((long long)r[2] * (s.R[1] << L) + r[1]) * (s.R[0] << L) + r[0]];
+}
 
 // Time / dt bookkeeping kept on the device so that a whole cycle (and a
 // multi-cycle hipGraph) runs without host round trips.  Mirrors

@@ -1,0 +1,325 @@
+// kr_smr.h -- the cross-level kernels of a statically refined mesh: the
+// halo fill where levels meet, the flux correction of faces whose
+// neighbour is finer, and the edge-EMF restriction with write-back.
+//
+// Semantics (the reference's act_bnd / act_flx / intg_flx where d_lvl != 0,
+// hydro/boundary/comm_functions.h:329-544, mhd/boundary/comm_functions.h
+// :467-954, hydro/integrator/integrator.h:313-382):
+//   * a halo cell covered by a finer leaf holds the volume-weighted mean of
+//     the 2x2x2 finer cells; one covered by a coarser leaf holds the coarser
+//     cell plus minmod-limited slopes (dq = minmod(q_r - q_c, q_c - q_l)/4
+//     per axis, signed by the half of the coarser cell it sits in).  Both
+//     sum in x, y, z order, so every block sees the same value at a given
+//     position; restricted halos are filled first and the prolongation's
+//     slopes read them (act_bnd's q_ghst stage before q_plgt);
+//   * a face whose neighbour is finer takes the area-weighted mean of the
+//     four finer face fluxes; the regular update skipped it and
+//     k_smr_intg adds it to the adjacent cell (z sides, then y, then x);
+//   * an edge of a coarser block that a finer leaf shares takes the
+//     length-weighted mean of the two finer edges along it; then the finer
+//     blocks' edge pairs on such lines are overwritten with that mean.
+#pragma once
+
+#include "kr_staged.h"
+
+namespace kr {
+
+constexpr int kSmrThreads = 256;
+
+template <class FT>
+__device__ __forceinline__ FT minmod_(FT a, FT b) {   // utilities/functions.h:159-170
+    return ((a < 0) ^ (b < 0)) ? (FT)0 : (kfabs(a) < kfabs(b) ? a : b);
+}
+
+// Wrap a global cell index of level L along the periodic axes.
+__device__ __forceinline__ void smr_wrap(const SmrView &s, const int *n, int L, int *G) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int N = (s.R[a] * n[a]) << L;
+        if (s.per[a] && (G[a] < 0 || G[a] >= N)) G[a] = ((G[a] % N) + N) % N;
+    }
+}
+
+// One task per grid.y: fill the halo block's cells of the owner's
+// neighbour region in direction `dir`.  prolong = 0: finer source
+// (restriction), 1: coarser source (prolongation).
+template <class FT, int NV>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_fill(MeshView<FT> m, SmrView s,
+                                                         const int4 *task, int prolong) {
+    const int4 tk = task[blockIdx.y];
+    const int hb = tk.x, b = tk.y, dir = tk.z;
+    const BlockGeo &g = m.geo[b];
+    if (m.ts[g.grp].skip) return;
+    int d[3] = {dir % 3 - 1, (dir / 3) % 3 - 1, dir / 9 - 1};
+    int ext[3], lo[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = d[a] == 0 ? m.n[a] : 2;
+        lo[a] = d[a] < 0 ? m.n[a] - 2 : 0;          // halo-block local origin
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ext[0] * ext[1] * ext[2]) return;
+    int hl[3] = {lo[0] + t % ext[0], lo[1] + (t / ext[0]) % ext[1], lo[2] + t / (ext[0] * ext[1])};
+    int G[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) G[a] = g.reg[a] * m.n[a] + hl[a] + d[a] * m.n[a];
+    smr_wrap(s, m.n, g.level, G);
+    FT out[NV];
+    if (!prolong) {
+        int r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = (2 * G[a]) / m.n[a];
+        const int f = smr_leaf(s, g.level + 1, r);
+        if (f < 0) return;                    // excluded by the host's topology check
+        const BlockGeo &gf = m.geo[f];
+        int l[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) l[a] = 2 * G[a] - gf.reg[a] * m.n[a];
+        const FT dv = (FT)gf.vol;
+        FT acc[NV], norm = 0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0;
+        const FT *W = m.W + (size_t)f * NV * m.ncell;
+#pragma unroll
+        for (int k0 = 0; k0 < 2; ++k0)
+#pragma unroll
+            for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+                for (int d2 = 0; d2 < 2; ++d2) {
+                    const size_t c = cidx(m.n, l[0] + k0, l[1] + d1, l[2] + d2);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] += W[(size_t)v * m.ncell + c] * dv;
+                    norm += dv;
+                }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v] = acc[v] / norm;
+    } else {
+        int r[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) r[a] = (G[a] >> 1) / m.n[a];
+        const int cb = smr_leaf(s, g.level - 1, r);
+        if (cb < 0) return;
+        const BlockGeo &gc = m.geo[cb];
+        int l[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) l[a] = (G[a] >> 1) - gc.reg[a] * m.n[a];
+        FT qc[NV], dq[3][NV];
+        load_cell<FT, NV>(m, cb, l, qc);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            FT qr[NV], ql[NV];
+            int p[3] = {l[0], l[1], l[2]};
+            ++p[a];
+            load_cell<FT, NV>(m, cb, p, qr);
+            p[a] -= 2;
+            load_cell<FT, NV>(m, cb, p, ql);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) dq[a][v] = minmod_<FT>(qr[v] - qc[v], qc[v] - ql[v]) * (FT)0.25f;
+        }
+        int sg[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) sg[a] = 2 * (G[a] & 1) - 1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            out[v] = qc[v] + dq[0][v] * (FT)sg[0] + dq[1][v] * (FT)sg[1] + dq[2][v] * (FT)sg[2];
+    }
+    FT *Wh = m.W + (size_t)hb * NV * m.ncell + cidx(m.n, hl[0], hl[1], hl[2]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Wh[(size_t)v * m.ncell] = out[v];
+}
+
+// Flux restriction onto a face whose neighbour is finer.  Task {block,
+// 2*axis + side}; one thread per coarse face.
+template <class FT, int NV>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrView s,
+                                                         const int2 *task) {
+    const int2 tk = task[blockIdx.y];
+    const int b = tk.x, a = tk.y >> 1, sd = tk.y & 1;
+    const BlockGeo &g = m.geo[b];
+    if (m.ts[g.grp].skip) return;
+    const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.n[a1] * m.n[a2]) return;
+    const int j1 = t % m.n[a1], j2 = t / m.n[a1];
+    int G[3];
+    G[a] = g.reg[a] * m.n[a] + (sd ? m.n[a] : -1);
+    G[a1] = g.reg[a1] * m.n[a1] + j1;
+    G[a2] = g.reg[a2] * m.n[a2] + j2;
+    smr_wrap(s, m.n, g.level, G);
+    int r[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) r[x] = (2 * G[x]) / m.n[x];
+    const int f = smr_leaf(s, g.level + 1, r);
+    if (f < 0) return;
+    const BlockGeo &gf = m.geo[f];
+    int l[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) l[x] = 2 * G[x] - gf.reg[x] * m.n[x];
+    l[a] = sd ? 0 : m.n[a];
+    const FT ds = (FT)gf.sfc[a];
+    FT fb[NV], norm = 0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fb[v] = 0;
+    const FT *Ff = m.F[a] + (size_t)f * NV * m.nfa[a];
+#pragma unroll
+    for (int d1 = 0; d1 < 2; ++d1)
+#pragma unroll
+        for (int d2 = 0; d2 < 2; ++d2) {
+            int p[3] = {l[0], l[1], l[2]};
+            p[a1] += d1;
+            p[a2] += d2;
+            const size_t x = xidx(m.n, a, p);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) fb[v] += Ff[(size_t)v * m.nfa[a] + x] * ds;
+            norm += ds;
+        }
+    int q[3];
+    q[a] = sd ? m.n[a] : 0;
+    q[a1] = j1;
+    q[a2] = j2;
+    FT *Fc = m.F[a] + (size_t)b * NV * m.nfa[a] + xidx(m.n, a, q);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Fc[(size_t)v * m.nfa[a]] = fb[v] / norm;
+}
+
+// intg_ker: add the corrected flux of every refined face to the adjacent
+// cell of the target slot, z sides first, then y, then x.  One thread per
+// cell of the listed blocks (interior cells return at once).
+template <class FT>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_intg(MeshView<FT> m, const int *blocks,
+                                                         int step) {
+    const int b = blocks[blockIdx.y];
+    const BlockGeo &g = m.geo[b];
+    if (m.ts[g.grp].skip) return;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.ncell) return;
+    const int i[3] = {(int)(c % m.n[0]), (int)((c / m.n[0]) % m.n[1]),
+                      (int)(c / ((size_t)m.n[0] * m.n[1]))};
+    const double dt = m.ts[g.grp].dt_use * (step == 0 ? 1.0 : 0.5);
+    double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * m.ncell + c;
+    for (int a = 2; a >= 0; --a)
+        for (int sd = 0; sd <= 1; ++sd) {
+            if (!((g.rface >> (2 * a + sd)) & 1)) continue;
+            if (i[a] != (sd ? m.n[a] - 1 : 0)) continue;
+            int q[3] = {i[0], i[1], i[2]};
+            q[a] += sd;
+            const FT *F = m.F[a] + (size_t)b * m.nv * m.nfa[a] + xidx(m.n, a, q);
+            const FT ds = (FT)g.sfc[a];
+            const FT dv = (FT)g.vol * (FT)(sd ? -1 : 1);
+            const double conv = dt * (double)ds / (double)dv;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Ut[(size_t)v * m.ncell] += conv * (double)F[(size_t)v * m.nfa[a]];
+        }
+}
+
+// Levels of the four cells around edge (component a) at local position i:
+// bit 0 set if a finer leaf shares it (first such cell in fo1/fo2), bit 1 if
+// a coarser one does.
+template <class FT>
+__device__ __forceinline__ int edge_levels(const MeshView<FT> &m, const BlockGeo &g, int a,
+                                           const int *i, int &fo1, int &fo2) {
+    const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+    int has = 0;
+    fo1 = fo2 = 0;
+    for (int o2 = -1; o2 <= 0; ++o2)
+        for (int o1 = -1; o1 <= 0; ++o1) {
+            int cc[3] = {i[0], i[1], i[2]};
+            cc[a1] += o1;
+            cc[a2] += o2;
+            int d[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) d[x] = cc[x] < 0 ? -1 : (cc[x] >= m.n[x] ? 1 : 0);
+            const int lv = (d[0] | d[1] | d[2]) ? g.lev[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)] : 0;
+            if (lv == 1 && !(has & 1)) {
+                has |= 1;
+                fo1 = o1;
+                fo2 = o2;
+            }
+            if (lv == -1) has |= 2;
+        }
+    return has;
+}
+
+template <class FT>
+__device__ __forceinline__ bool on_block_surface(const MeshView<FT> &m, const int *i) {
+    bool s = false;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) s |= i[x] == 0 || i[x] == m.n[x];
+    return s;
+}
+
+// Coarser side: edges shared with a finer leaf take the length-weighted
+// mean of the two finer edges (mhd/boundary/comm_functions.h:795-864 and
+// :919-954, the packed value).  Reads only finer blocks' edges, writes only
+// this block's: one launch is race-free.
+template <class FT>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_emf_coarse(MeshView<FT> m, SmrView s,
+                                                               const int *blocks) {
+    const int b = blocks[blockIdx.y];
+    const BlockGeo &g = m.geo[b];
+    if (m.ts[g.grp].skip) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nface) return;
+    const int i[3] = {(int)(t % (m.n[0] + 1)), (int)((t / (m.n[0] + 1)) % (m.n[1] + 1)),
+                      (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)))};
+    if (!on_block_surface(m, i)) return;
+    for (int a = 0; a < 3; ++a) {
+        if (i[a] >= m.n[a]) continue;
+        int o1, o2;
+        if (!(edge_levels(m, g, a, i, o1, o2) & 1)) continue;
+        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+        int G[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) G[x] = g.reg[x] * m.n[x] + i[x];
+        G[a1] += o1;
+        G[a2] += o2;
+        smr_wrap(s, m.n, g.level, G);
+        int r[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) r[x] = (2 * G[x]) / m.n[x];
+        const int f = smr_leaf(s, g.level + 1, r);
+        if (f < 0) continue;
+        const BlockGeo &gf = m.geo[f];
+        int e[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) e[x] = 2 * G[x] - gf.reg[x] * m.n[x];
+        e[a1] += o1 < 0 ? 2 : 0;
+        e[a2] += o2 < 0 ? 2 : 0;
+        const FT *EEf = m.EE + ((size_t)f * 3 + a) * m.nface;
+        const FT dl = (FT)gf.dx[a];
+        const FT el = EEf[fidx(m.n, e[0], e[1], e[2])] * dl;
+        ++e[a];
+        const FT er = EEf[fidx(m.n, e[0], e[1], e[2])] * dl;
+        m.EE[((size_t)b * 3 + a) * m.nface + t] = (el + er) / (dl + dl);
+    }
+}
+
+// Finer side: edge pairs on a coarser edge line shared with a coarser leaf
+// take their length-weighted mean (the write-back of comm_functions.h
+// :855-861, 950-952).  Runs after k_smr_emf_coarse read the originals.
+template <class FT>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_emf_wb(MeshView<FT> m, const int *blocks) {
+    const int b = blocks[blockIdx.y];
+    const BlockGeo &g = m.geo[b];
+    if (m.ts[g.grp].skip) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nface) return;
+    const int i[3] = {(int)(t % (m.n[0] + 1)), (int)((t / (m.n[0] + 1)) % (m.n[1] + 1)),
+                      (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)))};
+    if (!on_block_surface(m, i)) return;
+    const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
+    for (int a = 0; a < 3; ++a) {
+        if (i[a] >= m.n[a] || (i[a] & 1)) continue;
+        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+        if ((i[a1] & 1) || (i[a2] & 1)) continue;
+        int o1, o2;
+        if (!(edge_levels(m, g, a, i, o1, o2) & 2)) continue;
+        FT *E = m.EE + ((size_t)b * 3 + a) * m.nface + t;
+        const FT dl = (FT)g.dx[a];
+        const FT avg = (E[0] * dl + E[sh[a]] * dl) / (dl + dl);
+        E[0] = avg;
+        E[sh[a]] = avg;
+    }
+}
+
+}  // namespace kr

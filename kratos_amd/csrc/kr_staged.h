@@ -114,7 +114,21 @@ __device__ __forceinline__ double load_bf(const MeshView<FT> &m, int b, int slot
     }
     int blk = b;
     if (out) {
-        blk = m.geo[b].nbr[nbr_dir(m, d)];
+        const BlockGeo &g = m.geo[b];
+        if (g.xface) {
+            // mhd/integrator/data.h:108-140: a transverse ghost line next to a
+            // cross-level face takes the normal field from the cell value of w
+            int bit = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (d[a]) bit = 1 << (2 * a + (d[a] > 0));
+            if (g.xface & bit) {
+                FT w[8];
+                load_cell<FT, 8>(m, b, pin, w);
+                return (double)w[BCC + c];
+            }
+        }
+        blk = g.nbr[nbr_dir(m, d)];
 #pragma unroll
         for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
     }
@@ -371,22 +385,28 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
             ++p1[ax];
             const size_t x0 = xidx(m.n, ax, i), x1 = xidx(m.n, ax, p1);
             const FT *F = m.F[ax] + (size_t)b * NV * m.nfa[ax];
+            // faces whose neighbour is finer are skipped here and added with
+            // the restricted flux afterwards (integrator.h:156-182 null_l/r)
+            const bool n0 = ((g.rface >> (2 * ax)) & 1) && i[ax] == 0;
+            const bool n1 = ((g.rface >> (2 * ax + 1)) & 1) && i[ax] + 1 == m.n[ax];
             if constexpr (std::is_same_v<FT, double>) {
                 const double fa = g.sfc[ax] * dt_dv;
+                const double fa0 = n0 ? -1 : fa, fa1 = n1 ? -1 : fa;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     double res(0);
-                    if (fa > 0) res += fa * F[(size_t)v * m.nfa[ax] + x0];
-                    if (fa > 0) res -= fa * F[(size_t)v * m.nfa[ax] + x1];
+                    if (fa0 > 0) res += fa0 * F[(size_t)v * m.nfa[ax] + x0];
+                    if (fa1 > 0) res -= fa1 * F[(size_t)v * m.nfa[ax] + x1];
                     uu[v] += res;
                 }
             } else {
                 const float fa = (float)(g.sfc[ax] * dt_dv);
+                const float fa0 = n0 ? -1.f : fa, fa1 = n1 ? -1.f : fa;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     double res(0);
-                    if (fa > 0) res += (double)(fa * F[(size_t)v * m.nfa[ax] + x0]);
-                    if (fa > 0) res -= (double)(fa * F[(size_t)v * m.nfa[ax] + x1]);
+                    if (fa0 > 0) res += (double)(fa0 * F[(size_t)v * m.nfa[ax] + x0]);
+                    if (fa1 > 0) res -= (double)(fa1 * F[(size_t)v * m.nfa[ax] + x1]);
                     uu[v] += res;
                 }
             }

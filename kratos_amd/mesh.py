@@ -314,9 +314,33 @@ class RefinedMeshSpec:
                 d.reg[a] = self.reg(b)[a]
                 d.n_ceff[a] = self.n_cell_block[a]
                 d.xf0[a], d.dx0[a] = xf0[a], dx0[a]
+            nb = self.neighbours(b)
             for k in range(27):
-                d.nbr[k] = -1
+                d.nbr[k] = nb[k]
         return arr
+
+    def neighbours(self, b):
+        """Same-level neighbour per direction (-1 where the neighbour region
+        is at another level or outside a non-periodic side); the library
+        derives the cross-level topology itself from level and reg."""
+        if not hasattr(self, "_where"):
+            self._where = {leaf: i for i, leaf in enumerate(self.leaves)}
+        L, r = self.level(b), self.reg(b)
+        out = [-1] * 27
+        for k in range(27):
+            d = (k % 3 - 1, (k // 3) % 3 - 1, k // 9 - 1)
+            q = []
+            for a in range(3):
+                v = r[a] + d[a]
+                lim = self.nblk[a] << L
+                if v < 0 or v >= lim:
+                    if not self.periodic[a]:
+                        break
+                    v %= lim
+                q.append(v)
+            else:
+                out[k] = self._where.get((L,) + tuple(q), -1)
+        return out
 
     def split(self, u, bfs=None):
         raise TypeError("refined meshes take per-block arrays (upload_blocks)")
