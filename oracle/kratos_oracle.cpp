@@ -657,21 +657,22 @@ struct Mesh {
                     for (int dx = -1; dx <= 1; ++dx) {
                         const int d[3] = {dx, dy, dz};
                         if (!dx && !dy && !dz) continue;
-                        int G[3];
-                        for (int a = 0; a < 3; ++a)
-                            G[a] = (b.reg[a] + d[a]) * b.n[a] + (d[a] < 0 ? b.n[a] - 1 : 0);
+                        // the ghost cells in direction d (two deep across, the
+                        // block's extent along) must lie on one level
                         int &lv = b.lev_nb[dir_index(d)];
-                        if (!wrap(b.level, G)) { lv = kOut; continue; }
-                        int src;
-                        lv = cover(b.level, G, src);
-                        // the whole neighbour region must be one level
                         for (int c = 0; c < 8; ++c) {
                             int H[3];
-                            for (int a = 0; a < 3; ++a)
-                                H[a] = (b.reg[a] + d[a]) * b.n[a] + ((c >> a) & 1) * (b.n[a] - 1);
-                            wrap(b.level, H);
+                            for (int a = 0; a < 3; ++a) {
+                                const int o = (c >> a) & 1;
+                                H[a] = b.reg[a] * b.n[a] + (d[a] < 0 ? -2 + o
+                                                           : d[a] > 0 ? b.n[a] + o
+                                                                      : o * (b.n[a] - 1));
+                            }
+                            if (!wrap(b.level, H)) { lv = kOut; break; }
                             int s2;
-                            if (cover(b.level, H, s2) != lv)
+                            const int l2 = cover(b.level, H, s2);
+                            if (c == 0) lv = l2;
+                            else if (l2 != lv)
                                 throw std::runtime_error("SMR: mesh is not 2:1 balanced");
                         }
                     }
