@@ -127,6 +127,40 @@ void meshgen::base_t::read(const input &args) {
     bnd_kinds.clear();
     args("boundary", "kinds", bnd_kinds);
     if (bnd_kinds.size() < 6) throw std::runtime_error("[boundary] kinds needs 6 entries");
+    // meshgen.cpp:111-135 load_refinement
+    ref_regions.clear();
+    for (const auto &p : args.get_prefixes()) {
+        if (p.find("refine_region") == std::string::npos) continue;
+        ref_region_t r;
+        r.level = args.get<int>(p, "level", 0);
+        type::coord2_t x[2];
+        for (int a = 0; a < 3; ++a) x[0][a] = 0;
+        args(p, "loc", x[0]);
+        args(p, "x_min", x[0]);
+        x[1] = x[0];
+        args(p, "x_max", x[1]);
+        for (int l = 0; l < 2; ++l) {
+            for (int a = 0; a < n_dim; ++a) {
+                r.i[l][a] = search_blk_idx(x[l][a], r.level, a);
+                if (r.i[l][a] < 0) throw std::runtime_error("Incorrect SMR region");
+            }
+            for (int a = n_dim; a < 3; ++a) r.i[l][a] = 0;
+        }
+        ref_regions.push_back(r);
+    }
+}
+
+int meshgen::base_t::search_blk_idx(double loc, int lvl, int axis) const {
+    if (loc > x_max[axis] || loc < x_min[axis]) return -1;
+    const int i_tot = n_logic[axis] << lvl;
+    int i_r = i_tot, i_l = 0;
+    const double i_ftot(i_tot);
+    while (i_r - i_l > 1) {
+        const int i_m = (i_l + i_r) / 2;
+        if (location(i_m / i_ftot, axis) < loc) i_l = i_m;
+        else i_r = i_m;
+    }
+    return i_l;
 }
 
 // ------------------------------------------------------------------ mesh_t
@@ -170,6 +204,9 @@ void mesh_t::build_blocks() {
     bool periodic[3];
     for (int a = 0; a < 3; ++a) periodic[a] = g.bnd_kinds[2 * a].find("per") != std::string::npos;
     blocks_.clear();
+    bool refined = false;
+    for (auto &r : g.ref_regions) refined |= r.level > 0;
+    if (refined) return build_refined(periodic);
     const int nb = g.n_logic.prod();
     blocks_.resize(nb);
     for (int b = 0; b < nb; ++b) {
@@ -206,6 +243,85 @@ void mesh_t::build_blocks() {
                     k.nbr[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)] =
                         ok ? (q[2] * g.n_logic[1] + q[1]) * g.n_logic[0] + q[0] : -1;
                 }
+    }
+}
+
+// A statically refined mesh: the reference's tree (tree.cpp:185-218
+// insert / fill_base_level, meshgen.cpp:137-158 generate_tree).  Inserting
+// a block masks its parent, inserts the parent's other children and the
+// parent's 26 same-level neighbours, so the leaves are 2:1 balanced.
+// Blocks are the leaves in the tree's order (level, then x, y, z), ids
+// 0..n-1 in that order (the reference's [mesh] dist_mode = 0 numbering);
+// the library derives the neighbours from level and logical index.
+void mesh_t::build_refined(const bool *periodic) {
+    const auto &g = *p_mgn;
+    typedef std::array<int, 4> key_t;            // level, x, y, z
+    std::map<key_t, bool> tree;                  // value: masked
+    auto lim = [&](int L, int a) { return a < g.n_dim ? (g.n_logic[a] << L) : 1; };
+    std::function<void(int, const int *)> insert = [&](int L, const int *r) {
+        const key_t k{L, r[0], r[1], r[2]};
+        if (tree.count(k)) return;
+        if (L > 0) {
+            int rc[3];
+            for (int a = 0; a < 3; ++a) rc[a] = a < g.n_dim ? (r[a] >= 0 ? r[a] / 2 : (r[a] - 1) / 2) : 0;
+            insert(L - 1, rc);
+            tree[key_t{L - 1, rc[0], rc[1], rc[2]}] = true;
+            for (int c = 0; c < (1 << g.n_dim); ++c) {
+                key_t ch{L, 0, 0, 0};
+                for (int a = 0; a < 3; ++a) ch[1 + a] = a < g.n_dim ? 2 * rc[a] + ((c >> a) & 1) : 0;
+                tree.emplace(ch, false);
+            }
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int d[3] = {dx, dy, dz};
+                        bool zero = true, skip = false;
+                        int q[3];
+                        for (int a = 0; a < 3; ++a) {
+                            zero &= d[a] == 0;
+                            if (a >= g.n_dim && d[a] != 0) skip = true;
+                            q[a] = rc[a] + d[a];
+                            const int m = lim(L - 1, a);
+                            if (a < g.n_dim && periodic[a]) q[a] = (q[a] + m) % m;
+                            if (q[a] < 0 || q[a] >= m) skip = true;
+                        }
+                        if (!zero && !skip) insert(L - 1, q);
+                    }
+        }
+        tree.emplace(k, false);
+    };
+    for (auto &rr : g.ref_regions) {
+        if (rr.level <= 0) continue;
+        for (int z = rr.i[0][2]; z <= rr.i[1][2]; ++z)
+            for (int y = rr.i[0][1]; y <= rr.i[1][1]; ++y)
+                for (int x = rr.i[0][0]; x <= rr.i[1][0]; ++x) {
+                    const int r[3] = {x, y, z};
+                    insert(rr.level, r);
+                }
+    }
+    for (int z = 0; z < g.n_logic[2]; ++z)
+        for (int y = 0; y < g.n_logic[1]; ++y)
+            for (int x = 0; x < g.n_logic[0]; ++x) tree.emplace(key_t{0, x, y, z}, false);
+    int id = 0;
+    for (auto &kv : tree) {
+        if (kv.second) continue;
+        block_t k;
+        k.id = id++;
+        k.reg.level = kv.first[0];
+        k.reg.reg = type::idx_t(kv.first[1], kv.first[2], kv.first[3]);
+        k.geo.n_ceff = g.n_block;
+        for (int a = 0; a < 3; ++a) {
+            const int n = g.n_block[a];
+            auto loc = [&](type::float2_t i) -> type::float2_t {
+                if (n == 1) return i;
+                i = (k.reg.reg[a] + i / n) / (type::float2_t)(g.n_logic[a] << k.reg.level);
+                return g.location(i, a);
+            };
+            k.geo.xf0[a] = (type::float_t)loc(0);
+            k.geo.dx0[a] = (type::float_t)(loc(1) - k.geo.xf0[a]);
+        }
+        for (int j = 0; j < 27; ++j) k.nbr[j] = -1;
+        blocks_.push_back(k);
     }
 }
 

@@ -13,6 +13,7 @@
 // the reference's error convention.
 #pragma once
 
+#include <array>
 #include <chrono>
 #include <functional>
 #include <map>
@@ -164,13 +165,24 @@ struct base_t {
 
 namespace meshgen {
 // Keys of the Cartesian generator (meshgen/meshgen.cpp:13-42).
+// Static refinement: [refine_region_*] sections with level and x_min /
+// x_max (or loc), meshgen.cpp:111-158.
 struct base_t {
     virtual ~base_t() {}
     type::coord2_t x_min, x_max;
     type::idx_t n_global, n_block, n_logic;
     int n_dim = 0;
     std::vector<std::string> bnd_kinds;
+    struct ref_region_t {
+        int level = 0;
+        int i[2][3];           // inclusive block-index box at that level
+    };
+    std::vector<ref_region_t> ref_regions;
     virtual void read(const input &args);
+    double location(double x_logic, int axis) const {
+        return x_min[axis] + x_logic * (x_max[axis] - x_min[axis]);
+    }
+    int search_blk_idx(double loc, int lvl, int axis) const;   // meshgen.cpp:355-374
 };
 }  // namespace meshgen
 
@@ -276,6 +288,7 @@ public:
 
 private:
     void build_blocks();
+    void build_refined(const bool *periodic);
     void pull_time();
     std::vector<std::function<void(const input &)>> reads_;
     std::map<int, std::function<void(const input &)>> inits_;

@@ -5,6 +5,8 @@ initial state each generator wrote.
 * sod: 512 x 2 x 1 hydro (PLM + HLLC + RK2, outflow, cfl 0.4) to t = 0.2
   with the t_lim top-up; the initial state must equal the Python Sod
   generator's and the final state the oracle's, bit for bit.
+* blast: MHD blast on a 3-level [refine_region_*] mesh; same tree as the
+  Python builder, final state bit-exact with the oracle.
 * vortex: 3-D Orszag-Tang-type MHD, 32^3 as 8 blocks of 16^3; n_cycle_lim
   = 9 runs 10 cycles (the reference's loop rule); final state bit-exact
   with the oracle, div B at roundoff.
@@ -133,3 +135,53 @@ def test_vortex_snapshot_restart(gpu_lib, tmp_path):
     from test_host_api import parse_args_map
     args0 = parse_args_map(s0["input_args"])
     assert args0["mesh|n_cell_global"].split() == ["32", "32", "32"]
+
+
+def test_blast_generator_refined(gpu_lib, tmp_path):
+    """The blast generator on its [refine_region_*] mesh: the host API builds
+    the same tree as RefinedMeshSpec (block count, order and geometry: the
+    initial state matches the Python blast on that spec), and the final state
+    equals the oracle's on that mesh bit for bit (n_cycle_lim = 5 runs 6
+    cycles)."""
+    from kratos_amd.mesh import RefinedMeshSpec
+    # one parameter file: like the reference's input::read (input.cpp:83),
+    # each file read resets the section list, so [refine_region_*] must sit
+    # in the last file given
+    par = tmp_path / "blast_dump.par"
+    par.write_text(open(os.path.join(HOST, "usr_example", "blast", "blast.par")).read()
+                   + f"\n[prob]\ndump = {tmp_path / 'blast.raw'}\n")
+    spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8),
+                           refine=[(1, (-0.25,) * 3, (0.2,) * 3), (2, (-0.05,) * 3, (0.05,) * 3)],
+                           x_min=(-0.5,) * 3, x_max=(0.5,) * 3)
+    exe = os.path.join(HOST, "bin", "blast")
+    print(f"\n[case] host pgen blast {par}", flush=True)
+    r = subprocess.run(["timeout", "-k", "10", "120", exe, str(par)], capture_output=True,
+                       text=True, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    raw, out = np.fromfile(tmp_path / "blast.raw", dtype=np.float64), r.stdout
+    assert f"blocks = {spec.n_blocks}" in out
+    n = spec.n_cell_block
+    nu, nf = n[0] * n[1] * n[2] * 8, (n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3
+    assert raw.size == 2 * spec.n_blocks * (nu + nf)
+    states = [[(raw[(s * spec.n_blocks + b) * (nu + nf):][:nu],
+                raw[(s * spec.n_blocks + b) * (nu + nf) + nu:][:nf])
+               for b in range(spec.n_blocks)] for s in range(2)]
+    py = problems.blast_blocks(spec)
+    for b in range(spec.n_blocks):
+        np.testing.assert_allclose(states[0][b][0], py[b][0], rtol=1e-14, atol=1e-15)
+    phys = kratos_amd.Physics(mhd=True, gamma=1.6666666666666667, cfl=0.3)
+    ref = oracle.OracleMesh(spec, phys, n_threads=16)
+    ref.upload_blocks(states[0])
+    assert ref.advance(6) == 6
+    fin = ref.download_blocks()
+    for b in range(spec.n_blocks):
+        np.testing.assert_array_equal(states[1][b][0].reshape(-1, 8)[:, :5],
+                                      fin[b][0].reshape(-1, 8)[:, :5])
+        f1 = states[1][b][1].reshape(n[2] + 5, n[1] + 5, n[0] + 5, 3)
+        f2 = fin[b][1].reshape(n[2] + 5, n[1] + 5, n[0] + 5, 3)
+        for c, sl in enumerate(((slice(2, n[2] + 2), slice(2, n[1] + 2), slice(2, n[0] + 3)),
+                                (slice(2, n[2] + 2), slice(2, n[1] + 3), slice(2, n[0] + 2)),
+                                (slice(2, n[2] + 3), slice(2, n[1] + 2), slice(2, n[0] + 2)))):
+            np.testing.assert_array_equal(f1[sl + (c,)], f2[sl + (c,)])
+    assert "max |div B|" in out
