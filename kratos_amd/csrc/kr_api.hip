@@ -149,6 +149,8 @@ struct kr_ctx {
     int nftask = 0;
     int *d_xblk = nullptr, *d_rblk = nullptr;   // blocks with cross-level / refined sides
     int nxblk = 0, nrblk = 0;
+    int *d_pure = nullptr;          // blocks with no cross-level direction (fused kernel)
+    int npure = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -189,13 +191,14 @@ void release(kr_ctx *c) {
     c->graph = nullptr;
     void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE,
                     c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
-                    c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk};
+                    c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk,
+                    c->d_pure};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->d_table = nullptr;
     c->d_fill[0] = c->d_fill[1] = nullptr;
     c->d_ftask = nullptr;
-    c->d_xblk = c->d_rblk = nullptr;
+    c->d_xblk = c->d_rblk = c->d_pure = nullptr;
     c->U = c->BF = nullptr;
     c->W = c->EE = nullptr;
     c->F[0] = c->F[1] = c->F[2] = nullptr;
@@ -256,8 +259,6 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     const size_t nb = c->descs.size();
     if (c->ngrp != 1) throw std::invalid_argument("refined meshes use one time group");
     if (c->ndim != 3) throw std::invalid_argument("refined meshes are 3-D");
-    if (c->cfg.kernel_path == KR_PATH_FUSED)
-        throw std::invalid_argument("refined meshes run the staged kernels");
     SmrBuild S;
     for (int a = 0; a < 3; ++a) {
         S.n[a] = c->n[a];
@@ -285,7 +286,7 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     }
     std::vector<int4> fill[2];
     std::vector<int2> ftask;
-    std::vector<int> xblk, rblk;
+    std::vector<int> xblk, rblk, pure;
     int nhalo = 0;
     for (size_t b = 0; b < nb; ++b) {
         const kr_block_desc &d = c->descs[b];
@@ -337,6 +338,7 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
             fin |= G.lev[k] == 1;
         }
         if (any) xblk.push_back((int)b);
+        else pure.push_back((int)b);
         if (G.rface) rblk.push_back((int)b);
         (void)fin;
     }
@@ -376,7 +378,6 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     }
     c->smr = true;
     c->nhalo = nhalo;
-    c->fused = false;
     for (int a = 0; a < 3; ++a) {
         c->sv.R[a] = S.R[a];
         c->sv.per[a] = S.per[a] ? 1 : 0;
@@ -405,6 +406,10 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     c->d_xblk = dalloc<int>(xblk.size());
     if (!xblk.empty())
         KR_HIP(hipMemcpy(c->d_xblk, xblk.data(), xblk.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->npure = (int)pure.size();
+    c->d_pure = dalloc<int>(pure.size());
+    if (!pure.empty())
+        KR_HIP(hipMemcpy(c->d_pure, pure.data(), pure.size() * sizeof(int), hipMemcpyHostToDevice));
     c->nrblk = (int)rblk.size();
     c->d_rblk = dalloc<int>(rblk.size());
     if (!rblk.empty())
@@ -536,7 +541,9 @@ void commit(kr_ctx *c) {
     // fused stage kernel: 3-D MHD blocks divisible by a supported tile
     // (kr_config.tile forces a shape); the variant is picked by timing at
     // the first step
-    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED && !c->smr) {
+    // (on a refined mesh the fused kernel takes the blocks with no
+    // cross-level direction; the others run the staged kernels)
+    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {
@@ -568,8 +575,8 @@ struct Launcher {
     bool tune = false;   // events around the fused stage-kernel launches only
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
 
-    template <class F> void run(int kind, F &&f) {
-        if (!prof && !(tune && kind == KR_K_UPDATE && c->fused)) {
+    template <class F> void run(int kind, F &&f, bool sweep_launch = false) {
+        if (!prof && !(tune && sweep_launch)) {
             f();
             return;
         }
@@ -583,23 +590,23 @@ struct Launcher {
     }
 
     template <class FT, int TX, int TY, int NT, int W>
-    void sweep_t(const MeshView<FT> &m, int step) {
-        const unsigned nb = (unsigned)c->descs.size();
+    void sweep_t(const MeshView<FT> &m, int step, unsigned nb) {
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
         hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W>), dim3(nt, nb), dim3(NT), 0, c->stream,
                            m, step);
     }
 
-    template <class FT> void sweep(const MeshView<FT> &m, int step, int variant) {
+    // grid rows: nb blocks (m.blist maps a row to its block when set)
+    template <class FT> void sweep(const MeshView<FT> &m, int step, int variant, unsigned nb) {
         constexpr bool f32 = std::is_same_v<FT, float>;
         switch (variant) {
-        case 0: sweep_t<FT, 32, 8, 512, 1>(m, step); break;
-        case 1: sweep_t<FT, 32, 8, 512, f32 ? 4 : 1>(m, step); break;
-        case 2: sweep_t<FT, 16, 16, 512, 1>(m, step); break;
-        case 3: sweep_t<FT, 16, 16, 512, f32 ? 4 : 1>(m, step); break;
-        case 4: sweep_t<FT, 16, 8, 256, 1>(m, step); break;
-        case 5: sweep_t<FT, 8, 8, 192, 1>(m, step); break;
-        default: sweep_t<FT, 16, 8, 256, f32 ? 4 : 1>(m, step); break;
+        case 0: sweep_t<FT, 32, 8, 512, 1>(m, step, nb); break;
+        case 1: sweep_t<FT, 32, 8, 512, f32 ? 4 : 1>(m, step, nb); break;
+        case 2: sweep_t<FT, 16, 16, 512, 1>(m, step, nb); break;
+        case 3: sweep_t<FT, 16, 16, 512, f32 ? 4 : 1>(m, step, nb); break;
+        case 4: sweep_t<FT, 16, 8, 256, 1>(m, step, nb); break;
+        case 5: sweep_t<FT, 8, 8, 192, 1>(m, step, nb); break;
+        default: sweep_t<FT, 16, 8, 256, f32 ? 4 : 1>(m, step, nb); break;
         }
     }
 
@@ -618,8 +625,8 @@ struct Launcher {
                                    c->ppb);
             });
         if constexpr (MHD) {
-            if (c->fused) {
-                run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant); });
+            if (c->fused && !c->smr) {
+                run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant, nb); }, true);
                 return;
             }
         }
@@ -640,24 +647,38 @@ struct Launcher {
                                            (const int4 *)c->d_fill[k], k);
                     });
         }
+        // staged kernels: every block, or on a refined mesh with the fused
+        // kernel the blocks that touch another level (the rest run fused)
+        MeshView<FT> ms = m;
+        unsigned ns = nb;
+        if (MHD && c->smr && c->fused) {
+            if (c->npure) {
+                MeshView<FT> mp = m;
+                mp.blist = c->d_pure;
+                run(KR_K_UPDATE, [&] { sweep<FT>(mp, step, c->variant, (unsigned)c->npure); }, true);
+            }
+            ms.blist = c->d_xblk;
+            ns = (unsigned)c->nxblk;
+        }
+        if (ns == 0) return;
         if (c->ndim > 2)
             run(KR_K_FLUX_Z, [&] {
-                hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), nb),
-                                   blk, 0, c->stream, m, step);
+                hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), ns),
+                                   blk, 0, c->stream, ms, step);
             });
         if (c->ndim > 1)
             run(KR_K_FLUX_Y, [&] {
-                hipLaunchKernelGGL((k_flux<FT, MHD, 1>), dim3(ceil_div(c->nfa[1], kThreads), nb),
-                                   blk, 0, c->stream, m, step);
+                hipLaunchKernelGGL((k_flux<FT, MHD, 1>), dim3(ceil_div(c->nfa[1], kThreads), ns),
+                                   blk, 0, c->stream, ms, step);
             });
         run(KR_K_FLUX_X, [&] {
-            hipLaunchKernelGGL((k_flux<FT, MHD, 0>), dim3(ceil_div(c->nfa[0], kThreads), nb), blk,
-                               0, c->stream, m, step);
+            hipLaunchKernelGGL((k_flux<FT, MHD, 0>), dim3(ceil_div(c->nfa[0], kThreads), ns), blk,
+                               0, c->stream, ms, step);
         });
         if constexpr (MHD)
             run(KR_K_EMF, [&] {
-                hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), nb), blk, 0,
-                                   c->stream, m);
+                hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), ns), blk, 0,
+                                   c->stream, ms);
             });
         if (c->smr) {                       // act_flx: EMF and flux restriction
             if (MHD && c->nxblk) {
@@ -682,8 +703,8 @@ struct Launcher {
         }
         run(KR_K_UPDATE, [&] {
             hipLaunchKernelGGL((k_update<FT, MHD>),
-                               dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), nb), blk, 0,
-                               c->stream, m, step);
+                               dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), ns), blk, 0,
+                               c->stream, ms, step);
         });
         if (c->smr && c->nrblk)             // intg_flx: the refined faces
             run(KR_K_UPDATE, [&] {

@@ -152,9 +152,9 @@ k_sweep(MeshView<FT> m, int step) {
     static_assert(S::NCOL <= NT, "one halo column per thread");
     static_assert(S::NZIP <= NT, "one z pencil per thread, in-plane items from a wave boundary");
     static_assert(TX * TY + TX + TY <= NT, "one owned cell / edge face per thread");
-    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
     const int tid = threadIdx.x;
-    const int b = blockIdx.y;
+    const int b = row_block(m);
     const int ntx = m.n[0] / TX;
     const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
     const int nz = m.n[2];

@@ -86,6 +86,7 @@ struct MeshView {
     FT *F[3];          // [block][var nv][nfa[a]]
     FT *EE;            // [block][comp 3][nface]                   (MHD)
     const BlockGeo *geo;
+    const int *blist;  // stage kernels: block of grid row y (nullptr: row y is block y)
     TimeState *ts;     // [group]
     double *dt_part;   // one CFL partial per c2p workgroup, block-major
 };

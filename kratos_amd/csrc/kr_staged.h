@@ -30,6 +30,12 @@ __device__ __forceinline__ size_t xidx(const int *n, int ax, const int *p) {
     return ((size_t)(p[2] + o2) * d1 + (p[1] + o1)) * d0 + (p[0] + o0);
 }
 
+// Block handled by grid row blockIdx.y of a stage kernel.
+template <class FT>
+__device__ __forceinline__ int row_block(const MeshView<FT> &m) {
+    return m.blist ? m.blist[blockIdx.y] : (int)blockIdx.y;
+}
+
 template <class FT>
 __device__ __forceinline__ int nbr_dir(const MeshView<FT> &m, const int *d) {
     return ((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1);
@@ -229,9 +235,9 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
 template <class FT, bool MHD, int AXIS>
 __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
     constexpr int axis = AXIS;
-    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
-    const int b = blockIdx.y;
+    const int b = row_block(m);
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.nfa[axis]) return;
     int D[3], o[3];
@@ -298,8 +304,8 @@ __device__ __forceinline__ FT flux_at(const MeshView<FT> &m, int b, int a, const
 // from the neighbour's primitives (:201-213); inside it equals the c2p ec.
 template <class FT>
 __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
-    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
-    const int b = blockIdx.y;
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
+    const int b = row_block(m);
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.nface) return;
     int i[3];
@@ -353,9 +359,9 @@ __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
 // (mhd/integrator/integrator.h:385-462 with rk2.h:191-205).
 template <class FT, bool MHD>
 __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
-    if (m.ts[m.geo[blockIdx.y].grp].skip) return;
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
     constexpr int NV = MHD ? 8 : 5;
-    const int b = blockIdx.y;
+    const int b = row_block(m);
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nitem = MHD ? m.nface : m.ncell;
     if (t >= nitem) return;

@@ -3,7 +3,9 @@ C-ABI against the CPU oracle, same per-block initial states.  FP64 form:
 every conserved value and face field bit-identical; mixed form: density L1
 <= 1e-5 relative.  Cases cover hydro and MHD, two and three levels, a
 physical (outflow / reflecting) hydro box, and a refined mesh whose blocks
-all sit on level 0 (the same-level path fed by the refined-mesh builder)."""
+all sit on level 0 (the same-level path fed by the refined-mesh builder).
+MHD runs the fused kernel on blocks with no cross-level direction and the
+staged kernels on the others (and everything staged when asked)."""
 import numpy as np
 import pytest
 
@@ -26,10 +28,10 @@ def _faces(spec, ba):
             f[2:n[2] + 3, 2:n[1] + 2, 2:n[0] + 2, 2])
 
 
-def _run(spec, phys, blocks, cycles, precision="fp64"):
+def _run(spec, phys, blocks, cycles, precision="fp64", path="auto"):
     print(f"\n[case] smr blocks={spec.n_blocks} levels={spec.n_levels} mhd={phys.mhd} "
-          f"prec={precision} cycles={cycles}", flush=True)
-    ctx = kratos_amd.Context(spec, phys, precision)
+          f"prec={precision} path={path} cycles={cycles}", flush=True)
+    ctx = kratos_amd.Context(spec, phys, precision, path=path)
     ctx.upload_blocks(blocks)
     done = ctx.advance(cycles)
     g = ctx.download_blocks()
@@ -52,11 +54,14 @@ def _assert_equal(spec, g, o, mhd):
                 np.testing.assert_array_equal(x, y, err_msg=f"block {b} faces")
 
 
-@pytest.mark.parametrize("mhd", [False, True])
-def test_smr_two_levels_fp64(gpu_lib, mhd):
+@pytest.mark.parametrize("mhd,path", [(False, "auto"), (True, "auto"), (True, "staged")])
+def test_smr_two_levels_fp64(gpu_lib, mhd, path):
+    """path auto: MHD blocks away from level changes run the fused kernel,
+    the others the staged kernels; staged: every block staged."""
     spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=TWO)
     phys = Physics(mhd=mhd, gamma=G, cfl=0.3)
-    g, o, gt, ot, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=5, mhd=mhd), 6)
+    g, o, gt, ot, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=5, mhd=mhd), 6,
+                              path=path)
     assert gt[0] == ot[0]
     _assert_equal(spec, g, o, mhd)
     if mhd:
