@@ -636,6 +636,10 @@ struct Mesh {
                 throw std::runtime_error("SMR blocks need an even size >= 4 per axis");
         nlev = 0;
         for (auto &b : blocks) nlev = std::max(nlev, b.level + 1);
+        if (cfg.mhd && nlev > 1)
+            for (int sd = 0; sd < 6; ++sd)
+                if (cfg.bnd[sd] != KRO_PER)
+                    throw std::runtime_error("refined MHD meshes are periodic");
         table.assign(nlev, std::vector<int>());
         for (int L = 0; L < nlev; ++L)
             table[L].assign((size_t)(nblk[0] << L) * (nblk[1] << L) * (nblk[2] << L), -1);
@@ -1122,6 +1126,104 @@ struct Mesh {
         }
     }
 
+    /* -------- MHD physical boundaries (mhd/boundary/phys.h:144-251,
+     * phys_functors.h:10-137).  One kernel per physical side, in the order
+     * x-, x+, y-, y+, z-, z+ (keeper.cpp:91-117); each covers its two ghost
+     * layers over the transverse range including the transverse ghost
+     * bands, reading sources clamped into the block along transverse axes
+     * that are physical on that end (f_i) and taken from the exchanged
+     * ghosts otherwise.  The primitives (with B_cc) and the face field get
+     * the functor's values: free / outflow copy the boundary cell (outflow
+     * zeroes an inflowing normal velocity first); reflect mirrors the cells
+     * with the normal velocity negated, B_cc's tangential components
+     * negated, and the face field mirrored at -i (tangential negated).
+     * The reference runs this once with the face field before the ghost
+     * exchange and once more (primitives only) after it; the face-field
+     * entries whose sources are exchanged ghosts are never read, so one
+     * pass after the exchange gives every value that is used.            */
+    void phys_block_mhd(Block &b, int step) {
+        const int nv = nvar();
+        const int n_gh = 2;
+        for (int side = 0; side < 6; ++side) {
+            const int axis = side >> 1;
+            const bool right = side & 1;
+            if (!b.phy[axis][right]) continue;
+            const int kind = cfg.bnd[side];
+            const int ap1 = mod3(axis + 1), ap2 = mod3(axis + 2);
+            const int n_ceff = b.n[axis];
+            auto f_i = [&](int a, int v, int i_m) {
+                if (b.phy[a][0]) v = std::max(v, 0);
+                if (b.phy[a][1]) v = std::min(v, i_m);
+                return v;
+            };
+            for (int t2 = -n_gh; t2 < b.n[ap2] + n_gh; ++t2)
+                for (int t1 = -n_gh; t1 < b.n[ap1] + n_gh; ++t1) {
+                    int i_tgt[3], i_src[3], i_sfc[3];
+                    i_tgt[ap1] = t1;
+                    i_tgt[ap2] = t2;
+                    i_src[ap1] = f_i(ap1, t1, b.n[ap1] - 1);
+                    i_src[ap2] = f_i(ap2, t2, b.n[ap2] - 1);
+                    i_sfc[ap1] = f_i(ap1, t1, b.n[ap1]);
+                    i_sfc[ap2] = f_i(ap2, t2, b.n[ap2]);
+                    double hyd[5][8], bc[5][3], bfb[5][3];   // buffer positions -2..2
+                    auto H = [&](int i) { return hyd[i + n_gh]; };
+                    auto C = [&](int i) { return bc[i + n_gh]; };
+                    auto F = [&](int i) { return bfb[i + n_gh]; };
+                    for (int i = 0; i <= n_gh; ++i) {
+                        i_src[axis] = right ? n_ceff - i - 1 : i;
+                        i_sfc[axis] = right ? n_ceff - i : i;
+                        const double *hs = b.w.at(i_src);
+                        const double *fs = b.bf.at(i_sfc, step);
+                        const double *fc = b.bf.at(i_src, step);
+                        for (int v = 0; v < nv; ++v) H(i)[v] = hs[v];
+                        for (int a = 0; a < 3; ++a) {
+                            C(i)[a] = hs[BCC + a];
+                            F(i)[a] = a == axis ? fs[a] : fc[a];
+                        }
+                    }
+                    const int i_mom = MOM + axis;
+                    if (kind == KRO_OUT || kind == KRO_FRE) {
+                        if (kind == KRO_OUT && H(0)[i_mom] * (right ? -1 : 1) > 0) H(0)[i_mom] = 0;
+                        for (int i = -n_gh; i < 0; ++i) {
+                            for (int v = 0; v < nv; ++v) H(i)[v] = H(0)[v];
+                            for (int a = 0; a < 3; ++a) {
+                                C(i)[a] = C(0)[a];
+                                F(i)[a] = F(0)[a];
+                            }
+                        }
+                    } else if (kind == KRO_REF) {
+                        for (int i = -n_gh; i < 0; ++i) {
+                            for (int v = 0; v < nv; ++v)
+                                H(i)[v] = H(-i - 1)[v] * (v == i_mom ? -1 : 1);
+                            for (int a = 0; a < 3; ++a) {
+                                if (a != axis) {
+                                    F(i)[a] = -F(-i)[a];
+                                    C(i)[a] = -C(-i - 1)[a];
+                                } else {
+                                    F(i)[a] = F(-i)[a];
+                                    C(i)[a] = C(-i - 1)[a];
+                                }
+                            }
+                        }
+                    }
+                    for (int i = -n_gh; i < 0; ++i) {
+                        i_tgt[axis] = right ? n_ceff - i - 1 : i;
+                        int i_fbc[3] = {i_tgt[0], i_tgt[1], i_tgt[2]};
+                        i_fbc[axis] = right ? n_ceff - i : i;
+                        double *ht = b.w.at(i_tgt);
+                        double *ff = b.bf.at(i_fbc, step);
+                        double *fc = b.bf.at(i_tgt, step);
+                        for (int v = 0; v < nv; ++v) ht[v] = H(i)[v];
+                        for (int a = 0; a < 3; ++a) {
+                            ht[BCC + a] = C(i)[a];
+                            if (a == axis) ff[a] = F(i)[a];
+                            else fc[a] = F(i)[a];
+                        }
+                    }
+                }
+        }
+    }
+
     /* -------- one pencil sweep along `axis` (prototypes/integrator.h
      * :14-32; hydro/integrator/{data,integrator}.h; mhd/integrator/
      * {data,integrator}.h).  Lines cover the interior and, for MHD, the
@@ -1286,6 +1388,27 @@ struct Mesh {
                     double *ee_g = b.ee.at(i);
                     for (int a = 0; a < 3; ++a)
                         if (i[a] < b.n[a]) ee_g[a] = ee[a];
+                    // boundary-plane advective edge EMF on a high out / fre
+                    // side (integrator.h:328-381): the interior cell's
+                    // -(v x B) with v = w[mom] / rho as written there
+                    for (int a = 0; a < 3; ++a) {
+                        if (i[a] != b.n[a] || b.n[a] < 2) continue;
+                        if (!b.phy[a][1]) continue;
+                        const int k = cfg.bnd[2 * a + 1];
+                        if (k != KRO_OUT && k != KRO_FRE) continue;
+                        int jc[3] = {i[0], i[1], i[2]};
+                        --jc[a];
+                        const double *wc = b.w.at(jc);
+                        const double irho = 1.0 / std::fmax(wc[RHO], 1e-30);
+                        double v_c[3], b_c[3];
+                        for (int q = 0; q < 3; ++q) {
+                            v_c[q] = wc[MOM + q] * irho;
+                            b_c[q] = wc[BCC + q];
+                        }
+                        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                        ee_g[a1] = v_c[a] * b_c[a2] - v_c[a2] * b_c[a];
+                        ee_g[a2] = v_c[a1] * b_c[a] - v_c[a] * b_c[a1];
+                    }
                 }
     }
 
@@ -1344,12 +1467,14 @@ struct Mesh {
                 exchange_block(bi, step);
             });
             if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
+            else par_blocks([&](int bi) { phys_block_mhd(blocks[bi], step); });
         } else {
             par_blocks([&](int bi) {
                 smr_ghost(bi, 1);
                 if (cfg.mhd) smr_bf_ghost(bi, step);
             });
             if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
+            else par_blocks([&](int bi) { phys_block_mhd(blocks[bi], step); });
             par_blocks([&](int bi) { smr_ghost(bi, 2); });
         }
         const double dt = dt_cur * (step == 0 ? 1.0 : 0.5);
@@ -1409,13 +1534,6 @@ static kro_mesh *kro_create_impl(const kro_config *c, bool do_build) {
                 g_err = "periodic must be set on both sides";
                 return nullptr;
             }
-        if (c->mhd)
-            for (int s = 0; s < 6; ++s)
-                if (c->bnd[s] != KRO_PER) {
-                    delete km;
-                    g_err = "oracle MHD supports periodic sides only";
-                    return nullptr;
-                }
         m.cfg.dt_init = c->dt_init; m.cfg.dt_expand = c->dt_expand;
         m.cfg.t_lim = c->t_lim; m.cfg.top_up = c->top_up != 0;
         m.cfg.nthreads = c->n_threads;
