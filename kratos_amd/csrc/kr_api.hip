@@ -486,8 +486,6 @@ void commit(kr_ctx *c) {
                 const int k = ((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1);
                 if (G.nbr[k] < 0) G.phy |= 1 << (2 * a + s);
             }
-        if (c->mhd && G.phy && !multi_level)
-            throw std::invalid_argument("MHD supports periodic / internal sides only");
     }
     if (multi_level) {
         build_smr(c, g);
@@ -502,7 +500,7 @@ void commit(kr_ctx *c) {
                         G.phy |= 1 << (2 * a + sd);
                 }
             if (c->mhd && G.phy)
-                throw std::invalid_argument("MHD supports periodic / internal sides only");
+                throw std::invalid_argument("refined MHD meshes are periodic");
         }
     }
     // pools: W also holds the halo blocks of a refined mesh
@@ -543,7 +541,13 @@ void commit(kr_ctx *c) {
     // the first step
     // (on a refined mesh the fused kernel takes the blocks with no
     // cross-level direction; the others run the staged kernels)
-    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
+    // MHD physical sides run the staged kernels (the fused plane loads read
+    // neighbours only)
+    bool phys_sides = false;
+    for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
+    if (c->mhd && phys_sides && c->cfg.kernel_path == KR_PATH_FUSED)
+        throw std::invalid_argument("MHD physical sides run the staged kernels");
+    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED && !phys_sides) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {

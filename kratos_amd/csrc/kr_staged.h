@@ -59,6 +59,7 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
     int blk = b;
     int flip = -1;      // reflect: negate v_ax;  outflow: clamp inflow of v_ax
     int clamp = -1;     // 2*ax + side of an outflow side
+    bool phys8 = false; // MHD cell behind a physical side
     if (out) {
         const int nb = m.geo[b].nbr[nbr_dir(m, d)];
         if (nb >= 0) {
@@ -85,11 +86,73 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) p[a] = p[a] < 0 ? 0 : (p[a] >= m.n[a] ? m.n[a] - 1 : p[a]);
+        } else {
+            phys8 = true;
+        }
+    }
+    int mflip = -1, mclamp = -1;       // MHD physical side: reflect axis / outflow side
+    if constexpr (NV == 8) {
+        if (phys8) {
+            // MHD physical side (mhd/boundary/phys.h:144-251): the cell is
+            // written last by the side of the highest out-of-range axis that
+            // is physical; its source is mirrored (reflect) or the boundary
+            // cell (free / outflow) along that axis, clamped into the block
+            // along other axes that are physical on that end, and taken from
+            // the neighbour along the remaining (periodic / internal) ones.
+            const BlockGeo &g = m.geo[b];
+            int al = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (d[a] != 0 && ((g.phy >> (2 * a + (d[a] > 0))) & 1)) al = a;
+            int q[3], e[3];
+            bool out2 = false;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int side = d[a] > 0;
+                q[a] = p[a];
+                if (a == al) {
+                    const int kind = m.bnd[2 * a + side];
+                    const int gd = side ? p[a] - m.n[a] : -1 - p[a];
+                    if (kind == 3) {
+                        q[a] = side ? m.n[a] - 1 - gd : gd;
+                        mflip = a;
+                    } else {
+                        q[a] = side ? m.n[a] - 1 : 0;
+                        if (kind == 1) mclamp = 2 * a + side;
+                    }
+                } else if (d[a] != 0 && ((g.phy >> (2 * a + side)) & 1)) {
+                    q[a] = side ? m.n[a] - 1 : 0;
+                }
+                e[a] = q[a] < 0 ? -1 : (q[a] >= m.n[a] ? 1 : 0);
+                out2 |= e[a] != 0;
+            }
+            if (out2) {
+                blk = g.nbr[nbr_dir(m, e)];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) q[a] -= e[a] * m.n[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] = q[a];
         }
     }
     const FT *src = m.W + (size_t)blk * NV * m.ncell + cidx(m.n, p[0], p[1], p[2]);
 #pragma unroll
     for (int v = 0; v < NV; ++v) w[v] = src[(size_t)v * m.ncell];
+    if constexpr (NV == 8) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (mflip == a) {
+                w[MOM + a] = w[MOM + a] * -1;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (c != a) w[BCC + c] = -w[BCC + c];
+            }
+            if (mclamp >= 0 && (mclamp >> 1) == a) {
+                const int side = mclamp & 1;
+                if (w[MOM + a] * (side ? -1 : 1) > 0) w[MOM + a] = 0;
+            }
+        }
+    }
     if constexpr (NV == 5) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -135,6 +198,29 @@ __device__ __forceinline__ double load_bf(const MeshView<FT> &m, int b, int slot
             }
         }
         blk = g.nbr[nbr_dir(m, d)];
+        if (blk < 0) {
+            // physical side (mhd/boundary/phys.h:178-229, phys_functors.h
+            // :10-137): the tangential face field of the ghost cell next to
+            // the side; free / outflow copy the boundary cell's, reflect
+            // takes minus the one at -i (two cells in).  Along the
+            // component's own axis the source row is clamped into the block
+            // where that axis is physical at the top.
+            int a = 0;
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                if (d[x]) a = x;
+            const int side = d[a] > 0;
+            const int kind = m.bnd[2 * a + side];
+            double sgn = 1;
+            if (kind == 3) {
+                p[a] = side ? m.n[a] - 2 : 1;
+                sgn = -1;
+            } else {
+                p[a] = side ? m.n[a] - 1 : 0;
+            }
+            if (((g.phy >> (2 * c + 1)) & 1) && p[c] >= m.n[c]) p[c] = m.n[c] - 1;
+            return sgn * m.BF[(((size_t)b * 2 + slot) * 3 + c) * m.nface + fidx(m.n, p[0], p[1], p[2])];
+        }
 #pragma unroll
         for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
     }
@@ -351,6 +437,30 @@ __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
         }
         m.EE[((size_t)b * 3 + ax) * m.nface + t] = ee;
     }
+    // boundary-plane advective edge EMF of a high out / fre side
+    // (mhd/integrator/integrator.h:328-381): the interior cell's value with
+    // v = w[mom] / rho as written there; later axes overwrite earlier ones
+    const BlockGeo &g = m.geo[b];
+    if (g.phy)
+        for (int a = 0; a < 3; ++a) {
+            if (i[a] != m.n[a] || m.n[a] < 2 || !((g.phy >> (2 * a + 1)) & 1)) continue;
+            const int k = m.bnd[2 * a + 1];
+            if (k != 1 && k != 2) continue;
+            int jc[3] = {i[0], i[1], i[2]};
+            --jc[a];
+            FT w[8];
+            load_cell<FT, 8>(m, b, jc, w);
+            const FT irho = (FT)1 / pmax(w[RHO], (FT)1e-30);
+            FT v_c[3], b_c[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                v_c[q] = w[MOM + q] * irho;
+                b_c[q] = w[BCC + q];
+            }
+            const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+            m.EE[((size_t)b * 3 + a1) * m.nface + t] = v_c[a] * b_c[a2] - v_c[a2] * b_c[a];
+            m.EE[((size_t)b * 3 + a2) * m.nface + t] = v_c[a1] * b_c[a] - v_c[a] * b_c[a1];
+        }
 }
 
 // ---------------------------------------------------------------- update
