@@ -202,7 +202,7 @@ k_sweep(MeshView<FT> m, int step) {
     };
     auto bfn = [&](int c, int fx, int fy, int fz) -> FT {
         const int p[3] = {x0 + fx, y0 + fy, fz};
-        return (FT)load_bf(m, b, step, c, p);
+        return (FT)load_bf_nbr(m, b, step, c, p);
     };
 
     // Asynchronous load of plane z of W into ring slot `slot` (no staging
