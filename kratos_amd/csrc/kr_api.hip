@@ -130,7 +130,7 @@ struct kr_ctx {
     size_t ncell = 0, nface = 0, nfa[3] = {0, 0, 0};
     // device buffers
     double *U = nullptr, *BF = nullptr;
-    void *W = nullptr, *F[3] = {nullptr, nullptr, nullptr}, *EE = nullptr;
+    void *W = nullptr, *F[3] = {nullptr, nullptr, nullptr}, *EE = nullptr, *EC = nullptr;
     BlockGeo *geo = nullptr;
     TimeState *ts = nullptr;        // [ngrp]
     int ngrp = 1;
@@ -177,6 +177,7 @@ struct kr_ctx {
         m.W = (FT *)W;
         for (int a = 0; a < 3; ++a) m.F[a] = (FT *)F[a];
         m.EE = (FT *)EE;
+        m.EC = (FT *)EC;
         m.geo = geo;
         m.ts = ts;
         m.dt_part = dt_part;
@@ -189,7 +190,7 @@ namespace {
 void release(kr_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
-    void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE,
+    void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE, c->EC,
                     c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
                     c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk,
                     c->d_pure};
@@ -200,7 +201,7 @@ void release(kr_ctx *c) {
     c->d_ftask = nullptr;
     c->d_xblk = c->d_rblk = c->d_pure = nullptr;
     c->U = c->BF = nullptr;
-    c->W = c->EE = nullptr;
+    c->W = c->EE = c->EC = nullptr;
     c->F[0] = c->F[1] = c->F[2] = nullptr;
     c->geo = nullptr;
     c->ts = nullptr;
@@ -565,6 +566,8 @@ void commit(kr_ctx *c) {
         c->tune_next = 0;
         c->tuned = c->candidates.size() <= 1;
     }
+    // cell-centred E for the staged EMF (c2p writes it only when allocated)
+    if (c->mhd && (!c->fused || c->smr)) KR_HIP(hipMalloc(&c->EC, nb * 3 * c->ncell * fs));
     c->stats.n_blocks = (int)nb;
     c->stats.n_cells = (long long)(nb * c->ncell);
     // 5 state passes per Heun cycle of (5 conserved + 3 face-B) FP64 values

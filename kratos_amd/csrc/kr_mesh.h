@@ -85,6 +85,8 @@ struct MeshView {
     FT *W;             // [block][var nv][ncell]
     FT *F[3];          // [block][var nv][nfa[a]]
     FT *EE;            // [block][comp 3][nface]                   (MHD)
+    FT *EC;            // [block][comp 3][cell] cell-centred E = B x v written by c2p
+                       // for the staged EMF (MHD; nullptr when only the fused kernel runs)
     const BlockGeo *geo;
     const int *blist;  // stage kernels: block of grid row y (nullptr: row y is block y)
     TimeState *ts;     // [group]

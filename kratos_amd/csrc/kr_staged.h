@@ -320,6 +320,14 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
             for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
 #pragma unroll
             for (int a = 0; a < 3; ++a) W[(size_t)(BCC + a) * m.ncell] = (FT)bc[a];
+            if (m.EC) {     // mhd/eos/adiabatic.h:214-219, from the stored values
+                const FT B[3] = {(FT)bc[0], (FT)bc[1], (FT)bc[2]};
+                const FT V[3] = {(FT)prim[MOM], (FT)prim[MOM + 1], (FT)prim[MOM + 2]};
+                FT *E = m.EC + (size_t)b * 3 * m.ncell + c;
+                E[0] = B[1] * V[2] - B[2] * V[1];
+                E[m.ncell] = B[2] * V[0] - B[0] * V[2];
+                E[2 * m.ncell] = B[0] * V[1] - B[1] * V[0];
+            }
             if (step == 0) dtc = dt_mhd(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl);
         } else {
             const double cs2 = c2p(cons, prim, e, hr, he);
@@ -399,6 +407,16 @@ __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
 // ------------------------------------------------------------------- EMF
 template <class FT>
 __device__ __forceinline__ FT ec_at(const MeshView<FT> &m, int b, const int *p, const int *ax) {
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) in &= p[a] >= 0 && p[a] < m.n[a];
+    if (in) {   // the block's own cell: the c2p value (equal to B x v of its w)
+        const size_t c = cidx(m.n, p[0], p[1], p[2]);
+        if (m.EC) return m.EC[((size_t)b * 3 + ax[0]) * m.ncell + c];
+        const FT *W = m.W + (size_t)b * 8 * m.ncell + c;
+        const size_t s = m.ncell;
+        return W[(BCC + ax[1]) * s] * W[(MOM + ax[2]) * s] - W[(BCC + ax[2]) * s] * W[(MOM + ax[1]) * s];
+    }
     FT w[8];
     load_cell<FT, 8>(m, b, p, w);
     return w[BCC + ax[1]] * w[MOM + ax[2]] - w[BCC + ax[2]] * w[MOM + ax[1]];
