@@ -34,7 +34,7 @@ enum kr_status {
 
 enum kr_model { KR_HYDRO = 0, KR_MHD = 1 };
 enum kr_precision { KR_FP64 = 0, KR_MIXED = 1 };
-enum kr_bnd { KR_PER = 0, KR_OUT = 1, KR_FRE = 2, KR_REF = 3 };
+enum kr_bnd { KR_PER = 0, KR_OUT = 1, KR_FRE = 2, KR_REF = 3, KR_FIX = 4 };
 
 typedef struct kr_config {
     int    model;          /* kr_model: PLM+HLLC+RK2 or PLM+HLLD+CT+RK2      */
@@ -51,6 +51,12 @@ typedef struct kr_config {
                               autotuner, which runs the first cycles with
                               one fitting variant each and keeps the
                               fastest (every variant gives the same state) */
+    double fix_w[5];       /* KR_FIX (the reference's "fixd") ghost state, in
+                              primitives: rho, p, v0, v1, v2 -- the [init]
+                              rho0 / pre0 / vel0 of hydro/boundary/
+                              keeper.cpp:60-72 */
+    double fix_b[3];       /* KR_FIX face and cell-centred B ([init] b0,
+                              mhd/boundary/keeper.h:126-146)              */
 } kr_config;
 
 enum kr_path { KR_PATH_AUTO = 0, KR_PATH_STAGED = 1, KR_PATH_FUSED = 2 };

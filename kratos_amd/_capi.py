@@ -11,10 +11,10 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkratos_a
 
 KR_HYDRO, KR_MHD = 0, 1
 KR_FP64, KR_MIXED = 0, 1
-KR_PER, KR_OUT, KR_FRE, KR_REF = 0, 1, 2, 3
+KR_PER, KR_OUT, KR_FRE, KR_REF, KR_FIX = 0, 1, 2, 3, 4
 KR_PATH_AUTO, KR_PATH_STAGED, KR_PATH_FUSED = 0, 1, 2
 PATHS = {"auto": KR_PATH_AUTO, "staged": KR_PATH_STAGED, "fused": KR_PATH_FUSED}
-BND_CODES = {"per": KR_PER, "out": KR_OUT, "fre": KR_FRE, "ref": KR_REF}
+BND_CODES = {"per": KR_PER, "out": KR_OUT, "fre": KR_FRE, "ref": KR_REF, "fixd": KR_FIX}
 
 KERNEL_KINDS = ["c2p", "dt_finalize", "flux_x", "flux_y", "flux_z", "emf", "update",
                 "cycle_begin"]  # the fused stage kernel reports under "update"
@@ -33,6 +33,8 @@ class KrConfig(C.Structure):
         ("n_cell_block", C.c_int * 3),
         ("kernel_path", C.c_int),
         ("tile", C.c_int * 3),
+        ("fix_w", C.c_double * 5),
+        ("fix_b", C.c_double * 3),
     ]
 
 

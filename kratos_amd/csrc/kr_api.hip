@@ -166,6 +166,8 @@ struct kr_ctx {
         m.nv = nv;
         for (int s = 0; s < 6; ++s) m.bnd[s] = cfg.bnd[s];
         m.gamma = cfg.gamma;
+        for (int k = 0; k < 5; ++k) m.fix_w[k] = cfg.fix_w[k];
+        for (int k = 0; k < 3; ++k) m.fix_b[k] = cfg.fix_b[k];
         m.cfl = cfg.cfl;
         m.d_floor = cfg.rho_floor;
         m.t_floor = cfg.tmp_floor;
@@ -853,7 +855,7 @@ int kr_create(const kr_config *cfg, kr_ctx **out) {
             why = "periodic must be set on both sides of an axis";
     }
     for (int s = 0; s < 6 && why.empty(); ++s)
-        if (cfg->bnd[s] < KR_PER || cfg->bnd[s] > KR_REF) why = "unknown boundary kind";
+        if (cfg->bnd[s] < KR_PER || cfg->bnd[s] > KR_FIX) why = "unknown boundary kind";
     if (!why.empty()) {
         delete c;
         return KR_EINVAL;

@@ -75,8 +75,9 @@ struct MeshView {
     int ndim;          // axes with more than one cell
     int nblocks;
     int nv;            // variables in W and F: 5 hydro, 8 MHD
-    int bnd[6];        // boundary kind per side (0 per, 1 out, 2 fre, 3 ref)
+    int bnd[6];        // boundary kind per side (0 per, 1 out, 2 fre, 3 ref, 4 fixd)
     double gamma, cfl, d_floor, t_floor;
+    double fix_w[5], fix_b[3];   // fixd ghost primitives and B
     size_t ncell;      // n0*n1*n2
     size_t nface;      // (n0+1)(n1+1)(n2+1): face / edge box
     size_t nfa[3];     // flux-box size for each sweep axis

@@ -60,6 +60,7 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
     int flip = -1;      // reflect: negate v_ax;  outflow: clamp inflow of v_ax
     int clamp = -1;     // 2*ax + side of an outflow side
     bool phys8 = false; // MHD cell behind a physical side
+    bool fixd = false;  // Dirichlet side: the ghost is the fixed state
     if (out) {
         const int nb = m.geo[b].nbr[nbr_dir(m, d)];
         if (nb >= 0) {
@@ -79,9 +80,10 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
                 if (kind == 3) {                       // reflect
                     p[a] = side ? m.n[a] - 1 - g : g;
                     flip = a;
-                } else {                               // outflow / free
+                } else {                               // outflow / free / fixd
                     p[a] = side ? m.n[a] - 1 : 0;
                     if (kind == 1) clamp = 2 * a + side;
+                    if (kind == 4) fixd = true;
                 }
             }
 #pragma unroll
@@ -119,6 +121,7 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
                     } else {
                         q[a] = side ? m.n[a] - 1 : 0;
                         if (kind == 1) mclamp = 2 * a + side;
+                        if (kind == 4) fixd = true;
                     }
                 } else if (d[a] != 0 && ((g.phy >> (2 * a + side)) & 1)) {
                     q[a] = side ? m.n[a] - 1 : 0;
@@ -161,6 +164,14 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
                 const int side = clamp & 1;
                 if (w[MOM + a] * (side ? -1 : 1) > 0) w[MOM + a] = 0;
             }
+        }
+    }
+    if (fixd) {             // hydro / mhd phys_functors.h fixed_t
+#pragma unroll
+        for (int v = 0; v < 5; ++v) w[v] = (FT)m.fix_w[v];
+        if constexpr (NV == 8) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) w[BCC + c] = (FT)m.fix_b[c];
         }
     }
 }
@@ -235,6 +246,7 @@ __device__ __forceinline__ double load_bf(const MeshView<FT> &m, int b, int slot
                 if (d[x]) a = x;
             const int side = d[a] > 0;
             const int kind = m.bnd[2 * a + side];
+            if (kind == 4) return m.fix_b[c];
             double sgn = 1;
             if (kind == 3) {
                 p[a] = side ? m.n[a] - 2 : 1;

@@ -180,6 +180,18 @@ void mesh_t::init(const input &args) {
     if (!p_com) enroll_comm<comm::base_t>();
     for (auto &r : reads_) r(args);
     input_args_ = args.get_item_map();
+    // "fixd" sides: the ambient [init] medium in primitives
+    // (hydro/boundary/keeper.cpp:60-72, mhd/boundary/keeper.h:126-146)
+    {
+        const auto v0 = args.get<type::coord2_t>("init", "vel0", 0);
+        const auto b0 = args.get<type::coord2_t>("init", "b0", 0);
+        fix_w_[0] = args.get<double>("init", "rho0", 0);
+        fix_w_[1] = args.get<double>("init", "pre0", 0);
+        for (int a = 0; a < 3; ++a) {
+            fix_w_[2 + a] = v0[a];
+            fix_b_[a] = b0[a];
+        }
+    }
     restart_ = args.found("file", "binary");
     p_cyc->init(args);
     if (restart_) {
@@ -330,6 +342,7 @@ static int bnd_code(const std::string &k) {
     if (k == "out") return KR_OUT;
     if (k == "fre") return KR_FRE;
     if (k == "ref") return KR_REF;
+    if (k == "fixd") return KR_FIX;
     throw std::runtime_error("Undef bnd " + k);
 }
 
@@ -344,6 +357,8 @@ void mesh_t::init_dynamics(int model, int precision, double gamma, double cfl, d
     c.rho_floor = rho_floor;
     c.tmp_floor = tmp_floor;
     for (int s = 0; s < 6; ++s) c.bnd[s] = bnd_code(p_mgn->bnd_kinds[s]);
+    for (int k = 0; k < 5; ++k) c.fix_w[k] = fix_w_[k];
+    for (int k = 0; k < 3; ++k) c.fix_b[k] = fix_b_[k];
     c.t_0 = p_cyc->t;
     c.t_lim = p_cyc->t_lim;
     c.dt_init = p_cyc->dt_init;

@@ -302,6 +302,7 @@ private:
     bool restart_ = false;
     std::string restart_file_;
     std::map<std::string, std::string> input_args_;
+    double fix_w_[5] = {0, 0, 0, 0, 0}, fix_b_[3] = {0, 0, 0};   // "fixd" ghost state
 };
 
 // Throws std::runtime_error carrying the C-ABI message when rc != KR_OK.

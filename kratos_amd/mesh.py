@@ -358,6 +358,10 @@ class Physics:
     dt_init: float = 0.0
     dt_expand: float = 2.0
     top_up: bool = False
+    # ghost state of "fixd" (Dirichlet) sides: primitives rho, p, v0, v1, v2
+    # and B -- the reference's [init] rho0 / pre0 / vel0 / b0
+    fix_w: tuple = (1.0, 1.0, 0.0, 0.0, 0.0)
+    fix_b: tuple = (0.0, 0.0, 0.0)
 
 
 class Context:
@@ -389,6 +393,10 @@ class Context:
         tile = tuple(tile) + (0,) * (3 - len(tile))
         for k in range(3):
             cfg.tile[k] = int(tile[k])
+        for k in range(5):
+            cfg.fix_w[k] = float(phys.fix_w[k])
+        for k in range(3):
+            cfg.fix_b[k] = float(phys.fix_b[k])
         h = C.c_void_p()
         rc = lib.kr_create(C.byref(cfg), C.byref(h))
         _capi.check(rc, None, "kr_create")

@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkratos_oracle.so")
 
-_BND = {"per": 0, "out": 1, "fre": 2, "ref": 3}
+_BND = {"per": 0, "out": 1, "fre": 2, "ref": 3, "fixd": 4}
 
 
 class KroConfig(C.Structure):
@@ -33,6 +33,8 @@ class KroConfig(C.Structure):
         ("dt_init", C.c_double), ("dt_expand", C.c_double), ("t_lim", C.c_double),
         ("top_up", C.c_int),
         ("n_threads", C.c_int),
+        ("fix_w", C.c_double * 5),
+        ("fix_b", C.c_double * 3),
     ]
 
 
@@ -101,6 +103,10 @@ class OracleMesh:
         c.dt_init, c.dt_expand, c.t_lim = phys.dt_init, phys.dt_expand, phys.t_lim
         c.top_up = int(phys.top_up)
         c.n_threads = int(n_threads)
+        for k in range(5):
+            c.fix_w[k] = float(getattr(phys, "fix_w", (1.0, 1.0, 0.0, 0.0, 0.0))[k])
+        for k in range(3):
+            c.fix_b[k] = float(getattr(phys, "fix_b", (0.0, 0.0, 0.0))[k])
         if phys.t_0 != 0.0:
             raise ValueError("oracle starts at t = 0")
         if hasattr(spec, "leaves"):             # statically refined mesh

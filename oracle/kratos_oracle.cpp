@@ -93,6 +93,7 @@ struct Cfg {
     double dt_init, dt_expand, t_lim;
     bool top_up;
     int nthreads;
+    double fix_w[5], fix_b[3];     // KRO_FIX ghost state (keeper.cpp:60-72)
 };
 
 struct Block {
@@ -1116,6 +1117,9 @@ struct Mesh {
                         for (int i = -n_gh; i < 0; ++i)
                             for (int n = 0; n < nv; ++n)
                                 B(i)[n] = B(-i - 1)[n] * (n == i_mom ? -1 : 1);
+                    } else if (kind == KRO_FIX) {       // phys_functors.h:25-48
+                        for (int i = -n_gh; i < 0; ++i)
+                            for (int n = 0; n < nv; ++n) B(i)[n] = cfg.fix_w[n];
                     }
                     for (int i = -n_gh; i < 0; ++i) {
                         i3[axis] = right ? b.n[axis] - i - 1 : i;
@@ -1189,6 +1193,15 @@ struct Mesh {
                             for (int a = 0; a < 3; ++a) {
                                 C(i)[a] = C(0)[a];
                                 F(i)[a] = F(0)[a];
+                            }
+                        }
+                    } else if (kind == KRO_FIX) {       // phys_functors.h:38-70 (MHD)
+                        for (int i = -n_gh; i < 0; ++i) {
+                            for (int v = 0; v < 5; ++v) H(i)[v] = cfg.fix_w[v];
+                            for (int a = 0; a < 3; ++a) {
+                                H(i)[BCC + a] = cfg.fix_b[a];
+                                C(i)[a] = cfg.fix_b[a];
+                                F(i)[a] = cfg.fix_b[a];
                             }
                         }
                     } else if (kind == KRO_REF) {
@@ -1537,6 +1550,8 @@ static kro_mesh *kro_create_impl(const kro_config *c, bool do_build) {
         m.cfg.dt_init = c->dt_init; m.cfg.dt_expand = c->dt_expand;
         m.cfg.t_lim = c->t_lim; m.cfg.top_up = c->top_up != 0;
         m.cfg.nthreads = c->n_threads;
+        for (int k = 0; k < 5; ++k) m.cfg.fix_w[k] = c->fix_w[k];
+        for (int k = 0; k < 3; ++k) m.cfg.fix_b[k] = c->fix_b[k];
         m.eos.gamma = c->gamma; m.eos.cfl = c->cfl;
         m.eos.d_floor = c->rho_floor; m.eos.t_floor = c->tmp_floor;
         m.eos.nhyd = c->mhd ? 8 : 5;

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-enum kro_bnd { KRO_PER = 0, KRO_OUT = 1, KRO_FRE = 2, KRO_REF = 3 };
+enum kro_bnd { KRO_PER = 0, KRO_OUT = 1, KRO_FRE = 2, KRO_REF = 3, KRO_FIX = 4 };
 
 typedef struct kro_config {
     int    mhd;               /* 0: hydro PLM+HLLC, 1: MHD PLM+HLLD+CT   */
@@ -33,6 +33,8 @@ typedef struct kro_config {
     double dt_init, dt_expand, t_lim;
     int    top_up;
     int    n_threads;         /* <= 0: all hardware threads               */
+    double fix_w[5];          /* KRO_FIX ghost primitives rho, p, v0..2    */
+    double fix_b[3];          /* KRO_FIX face / cell-centred B             */
 } kro_config;
 
 typedef struct kro_mesh kro_mesh;

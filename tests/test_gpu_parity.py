@@ -265,3 +265,20 @@ def test_mhd_physical_sides_mixed(gpu_lib, bnd, bar):
     u, b, _ = problems.turbulence((32, 32, 32), seed=13, beta=1.0)
     (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, precision="mixed")
     assert rel_l1(gu[0], ou[0]) <= bar
+
+
+@pytest.mark.parametrize("mhd", [False, True])
+def test_fixed_sides_fp64(gpu_lib, mhd):
+    """"fixd" (Dirichlet) sides holding an ambient state, hydro and MHD,
+    bit-identical with the oracle after 10 cycles."""
+    spec = kratos_amd.MeshSpec((32, 32, 32), (16, 16, 16),
+                               bnd=("fixd", "out", "per", "per", "fixd", "fixd"))
+    phys = kratos_amd.Physics(mhd=mhd, gamma=5.0 / 3.0, cfl=0.3,
+                              fix_w=(1.1, 0.7, 0.3, -0.2, 0.1), fix_b=(0.4, -0.3, 0.2))
+    u, b, _ = problems.turbulence((32, 32, 32), seed=14, beta=2.0)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b if mhd else None, 10)
+    assert gt[0] == ot[0]
+    np.testing.assert_array_equal(gu, ou)
+    if mhd:
+        for x, y in zip(gb, ob):
+            np.testing.assert_array_equal(x, y)

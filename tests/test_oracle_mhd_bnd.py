@@ -173,3 +173,27 @@ def test_block_decomposition_with_physical_sides():
     np.testing.assert_array_equal(res[0][0], res[1][0])
     for x, y in zip(res[0][1], res[1][1]):
         np.testing.assert_array_equal(x, y)
+
+
+def test_fixed_sides_hold_the_ambient_state():
+    """A "fixd" (Dirichlet) side holds the ambient state in its ghosts: a
+    uniform state equal to that ambient medium stays exactly uniform (hydro
+    and MHD), and a different interior state is driven by the ghosts (the
+    ghosts are the fixed state, not copies of the interior)."""
+    for mhd in (False, True):
+        spec = RefinedMeshSpec((16, 8, 8), (8, 8, 8), bnd=("fixd", "fixd", "per", "per", "fixd", "fre"))
+        amb_v, amb_b = (0.2, -0.1, 0.05), (0.3, 0.2, -0.1)
+        phys = Physics(mhd=mhd, gamma=G, cfl=0.3, fix_w=(1.0, 0.6, 0.2, -0.1, 0.05),
+                       fix_b=amb_b if mhd else (0.0, 0.0, 0.0))
+        blocks = problems.wave_blocks(spec, amp=0.0, vel=amb_v, pre=0.6, mhd=mhd, b0=amb_b)
+        o = oracle.OracleMesh(spec, phys, n_threads=2)
+        o.upload_blocks(blocks)
+        o.advance(4)
+        for (ua, _), (ub, _) in zip(o.download_blocks(), blocks):
+            np.testing.assert_allclose(ua, ub, rtol=1e-13, atol=1e-14)
+        other = problems.wave_blocks(spec, amp=0.0, vel=amb_v, pre=0.9, mhd=mhd, b0=amb_b)
+        o = oracle.OracleMesh(spec, phys, n_threads=2)
+        o.upload_blocks(other)
+        o.advance(4)
+        diff = max(float(np.abs(ua - ub).max()) for (ua, _), (ub, _) in zip(o.download_blocks(), other))
+        assert diff > 1e-3
