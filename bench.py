@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--ens-n", type=int, default=64)
     ap.add_argument("--t-ens", type=float, default=0.05)
     ap.add_argument("--no-smr", action="store_true")
+    ap.add_argument("--smr-only", action="store_true",
+                    help="run only the refined-mesh blast leg and print its JSON")
     ap.add_argument("--smr-root", type=int, default=128)
     ap.add_argument("--smr-block", type=int, default=16)
     ap.add_argument("--tile", default="",
@@ -301,6 +303,11 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.smr_only:
+        smr = smr_leg(args.smr_root, args.smr_block, max(args.steps // 2, 4), args.warmup, 20)
+        print(json.dumps({k: (float(v) if isinstance(v, np.floating) else v) for k, v in smr.items()}))
+        return
 
     import kratos_amd
     n, nb = args.n, args.block
