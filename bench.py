@@ -47,7 +47,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STATE_DOUBLES = 5 + 3          # conserved rho, E, m + one face B per axis
+# state per cell in the algorithmic byte count: the reference's conserved u
+# (rho, E, m, B_cc: 8 doubles) and one face B per axis (3); 5 passes of it
+# per Heun cycle = 440 B per MHD cell-update.  This build keeps no u[B_cc]
+# (c2p recomputes it from the face field), so it needs only 8 of the 11.
+STATE_DOUBLES = 8 + 3
 
 
 def alg_bytes(kind, fs, fused):
@@ -55,8 +59,9 @@ def alg_bytes(kind, fs, fused):
     flux kernels) of one kernel kind, averaged over the two RK substeps.
 
     fused stage kernel ("update" in the fused path): the state it must read
-    and write once -- substep 0 reads U, BF of slot 0 and writes U, BF of
-    slot 1 (2 x 8 doubles); substep 1 also reads the target slot (3 x 8).
+    and write once -- substep 0 reads slot 0 and writes slot 1 (2 passes of
+    11 doubles), substep 1 also reads the target slot (3 passes), i.e. the
+    440 B per cell-update of the cycle split over its two launches.
     Staged kernels: their own inputs and outputs read / written once.
     """
     if kind == "update" and fused:
