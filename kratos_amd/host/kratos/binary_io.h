@@ -92,6 +92,11 @@ public:
         }
     }
     bool has_tag(const std::string &tag) const { return items_.count(tag) > 0; }
+    std::vector<std::string> tags() const {
+        std::vector<std::string> t;
+        for (auto &kv : items_) t.push_back(kv.first);
+        return t;
+    }
     const std::vector<char> &bytes(const std::string &tag) const {
         auto it = items_.find(tag);
         if (it == items_.end()) throw std::runtime_error("snapshot has no tag " + tag);

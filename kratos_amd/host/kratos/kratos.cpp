@@ -203,7 +203,8 @@ void mesh_t::init(const input &args) {
         p_cyc->dt_init = seed;                 // the device seed the run stopped with
     }
     p_mgn->read(args);
-    build_blocks();
+    if (restart_) blocks_from_snapshot();
+    else build_blocks();
     for (auto &p : inits_) p.second(args);
     if (!ctx_) throw std::runtime_error("no dynamics module enrolled");
 }
@@ -335,6 +336,47 @@ void mesh_t::build_refined(const bool *periodic) {
         for (int j = 0; j < 27; ++j) k.nbr[j] = -1;
         blocks_.push_back(k);
     }
+}
+
+// A restart takes its blocks from the snapshot (the reference reads its
+// tree back from the file, meshgen.cpp:96-103): every block_<id>|level and
+// |i_logic pair, in id order, with the geometry of that level.
+void mesh_t::blocks_from_snapshot() {
+    const auto &g = *p_mgn;
+    binary_io::base_t bio;
+    bio.load(restart_file_);
+    std::map<int, block_t> found;
+    for (const auto &t : bio.tags()) {
+        const std::string suf = "|level";
+        if (t.rfind("block_", 0) != 0 || t.size() <= suf.size() ||
+            t.compare(t.size() - suf.size(), suf.size(), suf) != 0)
+            continue;
+        const int id = std::stoi(t.substr(6, t.size() - suf.size() - 6));
+        const std::string pre = "block_" + std::to_string(id) + "|";
+        block_t k;
+        k.id = id;
+        bio.read(&k.reg.level, 1, pre + "level");
+        bio.read(k.reg.reg.x, 3, pre + "i_logic");
+        k.geo.n_ceff = g.n_block;
+        for (int a = 0; a < 3; ++a) {
+            const int n = g.n_block[a];
+            auto loc = [&](type::float2_t i) -> type::float2_t {
+                if (n == 1) return i;
+                i = (k.reg.reg[a] + i / n) / (type::float2_t)(g.n_logic[a] << k.reg.level);
+                return g.location(i, a);
+            };
+            k.geo.xf0[a] = (type::float_t)loc(0);
+            k.geo.dx0[a] = (type::float_t)(loc(1) - k.geo.xf0[a]);
+        }
+        for (int j = 0; j < 27; ++j) k.nbr[j] = -1;
+        found[id] = k;
+    }
+    if (found.empty()) throw std::runtime_error("snapshot holds no blocks: " + restart_file_);
+    bool refined = false;
+    for (auto &kv : found) refined |= kv.second.reg.level != 0;
+    if (!refined) return build_blocks();        // same-level: the Cartesian order and neighbours
+    blocks_.clear();
+    for (auto &kv : found) blocks_.push_back(kv.second);
 }
 
 static int bnd_code(const std::string &k) {

@@ -289,6 +289,7 @@ public:
 private:
     void build_blocks();
     void build_refined(const bool *periodic);
+    void blocks_from_snapshot();
     void pull_time();
     std::vector<std::function<void(const input &)>> reads_;
     std::map<int, std::function<void(const input &)>> inits_;

@@ -185,3 +185,36 @@ def test_blast_generator_refined(gpu_lib, tmp_path):
                                 (slice(2, n[2] + 3), slice(2, n[1] + 2), slice(2, n[0] + 2)))):
             np.testing.assert_array_equal(f1[sl + (c,)], f2[sl + (c,)])
     assert "max |div B|" in out
+
+
+def test_blast_refined_snapshot_restart(gpu_lib, tmp_path):
+    """A refined run restarted from its snapshot takes its blocks (level,
+    i_logic) from the file, as the reference reads its tree back, and ends
+    bit-identical to the uninterrupted run (the loop rule: n_cycle_lim = 9
+    runs 10 cycles; 4 runs 5 and stores cycle 4; the restart to 8 runs 5
+    more)."""
+    exe = os.path.join(HOST, "bin", "blast")
+    base = open(os.path.join(HOST, "usr_example", "blast", "blast.par")).read()
+
+    def run(args, text=None):
+        if text is not None:
+            o = tmp_path / f"par_{len(list(tmp_path.iterdir()))}.par"
+            o.write_text(text)
+            args = args + [str(o)]
+        print(f"\n[case] host pgen blast {args}", flush=True)
+        r = subprocess.run(["timeout", "-k", "10", "120", exe] + args, capture_output=True,
+                           text=True, cwd=tmp_path)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0, r.stderr
+
+    run([], base + f"\n[cycle]\nn_cycle_lim = 9\nfinal_output = 1\nprefix_output = {tmp_path}/A\n")
+    run([], base + f"\n[cycle]\nn_cycle_lim = 4\nfinal_output = 1\nprefix_output = {tmp_path}/B\n")
+    run([str(tmp_path / "B_00001.bin")], f"[cycle]\nn_cycle_lim = 8\nprefix_output = {tmp_path}/C\n")
+    a, c = _snap_fields(tmp_path / "A_00001.bin"), _snap_fields(tmp_path / "C_00002.bin")
+    assert a["time"] == c["time"]
+    blocks = sorted(k for k in a if k.endswith("|hydro_cons_field"))
+    assert len(blocks) > 64
+    levels = {np.frombuffer(a[k.replace("hydro_cons_field", "level")], "<i4")[0] for k in blocks}
+    assert levels == {0, 1, 2}
+    for k in blocks + [k.replace("hydro_cons", "field_bf") for k in blocks]:
+        np.testing.assert_array_equal(np.frombuffer(a[k], "<f8"), np.frombuffer(c[k], "<f8"))

@@ -118,3 +118,16 @@ def test_smr_mixed_l1(gpu_lib):
         den += np.abs(orr).sum() * v
     assert num / den <= 1e-5
     assert divb < 1e-11
+
+
+def test_unbalanced_mesh_rejected(gpu_lib):
+    """kr_add_blocks / the first upload refuses a block list whose levels
+    differ by two across a ghost region, with a message (no launch)."""
+    from test_oracle_smr import _unbalanced_spec
+    spec = _unbalanced_spec()
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    print("\n[case] unbalanced refined mesh (host-side rejection)", flush=True)
+    ctx = kratos_amd.Context(spec, phys, "fp64")
+    with pytest.raises(kratos_amd.KratosError, match="balanced|within one level"):
+        ctx.upload_blocks(problems.smooth_blocks(spec, seed=1))
+    ctx.close()

@@ -214,3 +214,22 @@ def test_uniform_state_on_refined_mesh():
         np.testing.assert_allclose(out[b][0], blocks[b][0], rtol=1e-13, atol=1e-14)
         for x, y in zip(_faces(spec, out[b][1]), _faces(spec, blocks[b][1])):
             np.testing.assert_allclose(x, y, rtol=1e-13, atol=1e-14)
+
+
+def _unbalanced_spec():
+    """Root 2^3 blocks of 8^3; root block (0,0,0) refined to level 1 and its
+    child (1,1,1) again to level 2: those level-2 blocks touch level-0
+    blocks across the root's faces (tree_t::insert would have refined the
+    neighbours; a hand-made list need not)."""
+    spec = RefinedMeshSpec((16, 16, 16), (8, 8, 8))
+    leaves = [(0, x, y, z) for x in range(2) for y in range(2) for z in range(2) if (x, y, z) != (0, 0, 0)]
+    leaves += [(1, x, y, z) for x in range(2) for y in range(2) for z in range(2) if (x, y, z) != (1, 1, 1)]
+    leaves += [(2, 2 + x, 2 + y, 2 + z) for x in range(2) for y in range(2) for z in range(2)]
+    spec.leaves = sorted(leaves)
+    spec.n_levels = 3
+    return spec
+
+
+def test_unbalanced_mesh_is_rejected():
+    with pytest.raises(RuntimeError, match="balanced|within one level"):
+        oracle.OracleMesh(_unbalanced_spec(), Physics(mhd=True, gamma=G), n_threads=1)
