@@ -13,9 +13,14 @@
  * mesh/types/dat_3d.h:153-156):
  *   u_aos : n0*n1*n2*nvar doubles, idx = ((k*n1 + j)*n0 + i)*nvar + v,
  *           nvar = 5 (hydro: rho, E, m0, m1, m2) or 8 (MHD: + Bcc0..2)
- *   bf_aos: N0*N1*N2*3 doubles, N[a] = n[a] + 1 + 4 (two ghost layers on
- *           each axis, mhd/types/fld_data.cpp:18-26),
- *           idx = (((k+2)*N1 + (j+2))*N0 + (i+2))*3 + c
+ *   bf_aos: N0*N1*N2*3 doubles, N[a] = n[a] + (n[a] > 1) + 4 (the faces
+ *           plus two ghost layers on each side, mhd/types/fld_data.cpp
+ *           :18-26: a thin, one-cell axis has a single face, both sides
+ *           of the cell), idx = (((k+2)*N1 + (j+2))*N0 + (i+2))*3 + c
+ *
+ * MHD blocks may be 2-D or 1-D: the one-cell (thin) axes must be the
+ * trailing ones (z, or y and z) and one cell wide over the whole mesh
+ * (mhd/integrator/integrator.h:221-227, 419-456).
  */
 #ifndef KRATOS_AMD_H
 #define KRATOS_AMD_H

@@ -483,10 +483,17 @@ void commit(kr_ctx *c) {
         }
         for (int a = 0; a < 3; ++a)
             for (int s = 0; s < 2; ++s) {
-                if (c->n[a] < 2) continue;
                 int dir[3] = {0, 0, 0};
                 dir[a] = s ? 1 : -1;
                 const int k = ((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1);
+                if (c->n[a] < 2) {
+                    // an MHD thin axis is one cell wide over the whole mesh: one
+                    // block layer, no neighbour but the block itself
+                    // (mhd/boundary/keeper.h:59-99)
+                    if (c->mhd && (d.reg[a] != 0 || (G.nbr[k] >= 0 && G.nbr[k] != (int)b)))
+                        throw std::invalid_argument("an MHD thin axis must be one block (one cell) wide");
+                    continue;
+                }
                 if (G.nbr[k] < 0) G.phy |= 1 << (2 * a + s);
             }
     }
@@ -848,9 +855,15 @@ int kr_create(const kr_config *cfg, kr_ctx **out) {
     else if (cfg->precision != KR_FP64 && cfg->precision != KR_MIXED) why = "unknown precision";
     else if (!(cfg->gamma > 1)) why = "gamma must exceed 1";
     else if (!(cfg->cfl > 0)) why = "cfl must be positive";
+    // MHD thin (one-cell) axes are the trailing ones, as the reference's
+    // EMF and CT assume (mhd/integrator/integrator.h:221-227)
+    if (c->mhd && why.empty()) {
+        if (cfg->n_cell_block[0] < 2) why = "MHD needs at least two cells along x";
+        else if (cfg->n_cell_block[1] < 2 && cfg->n_cell_block[2] >= 2)
+            why = "MHD thin axes must be the trailing ones";
+    }
     for (int a = 0; a < 3 && why.empty(); ++a) {
         if (cfg->n_cell_block[a] < 1) why = "n_cell_block must be >= 1";
-        if (c->mhd && cfg->n_cell_block[a] < 2) why = "MHD needs 3-D blocks";
         if ((cfg->bnd[2 * a] == KR_PER) != (cfg->bnd[2 * a + 1] == KR_PER))
             why = "periodic must be set on both sides of an axis";
     }
@@ -913,11 +926,15 @@ int kr_upload(kr_ctx *c, int id, const double *u_aos, const double *bf_aos) {
         KR_HIP(hipMemcpy(c->U + b * 2 * 5 * c->ncell, s.data(), 5 * c->ncell * sizeof(double),
                          hipMemcpyHostToDevice));
         if (c->mhd) {
-            const int N0 = c->n[0] + 5, N1 = c->n[1] + 5;
+            // host face array: n + (n > 1) faces per axis plus two ghost layers
+            // each side (mhd/types/fld_data.cpp:18-26); a thin axis has one
+            const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
+                               c->n[2] + (c->n[2] > 1)};
+            const int N0 = nf[0] + 4, N1 = nf[1] + 4;
             s.assign(3 * c->nface, 0.0);
-            for (int k = 0; k <= c->n[2]; ++k)
-                for (int j = 0; j <= c->n[1]; ++j)
-                    for (int i = 0; i <= c->n[0]; ++i) {
+            for (int k = 0; k < nf[2]; ++k)
+                for (int j = 0; j < nf[1]; ++j)
+                    for (int i = 0; i < nf[0]; ++i) {
                         const size_t src = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
                         const size_t f = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
                         for (int a = 0; a < 3; ++a) s[a * c->nface + f] = bf_aos[src + a];
@@ -947,7 +964,10 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
             std::vector<double> f(3 * c->nface);
             KR_HIP(hipMemcpy(f.data(), c->BF + b * 2 * 3 * c->nface, 3 * c->nface * sizeof(double),
                              hipMemcpyDeviceToHost));
-            const size_t sh[3] = {1, (size_t)(c->n[0] + 1), (size_t)(c->n[0] + 1) * (c->n[1] + 1)};
+            // a thin axis has one face, both sides of the cell
+            const size_t sh[3] = {c->n[0] > 1 ? (size_t)1 : 0,
+                                  c->n[1] > 1 ? (size_t)(c->n[0] + 1) : 0,
+                                  c->n[2] > 1 ? (size_t)(c->n[0] + 1) * (c->n[1] + 1) : 0};
             for (int k = 0; k < c->n[2]; ++k)
                 for (int j = 0; j < c->n[1]; ++j)
                     for (int i = 0; i < c->n[0]; ++i) {
@@ -958,11 +978,13 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
                                 (f[a * c->nface + f0] + f[a * c->nface + f0 + sh[a]]) / 2;
                     }
             if (bf_aos) {
-                const int N0 = c->n[0] + 5, N1 = c->n[1] + 5, N2 = c->n[2] + 5;
+                const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
+                                   c->n[2] + (c->n[2] > 1)};
+                const int N0 = nf[0] + 4, N1 = nf[1] + 4, N2 = nf[2] + 4;
                 std::memset(bf_aos, 0, (size_t)N0 * N1 * N2 * 3 * sizeof(double));
-                for (int k = 0; k <= c->n[2]; ++k)
-                    for (int j = 0; j <= c->n[1]; ++j)
-                        for (int i = 0; i <= c->n[0]; ++i) {
+                for (int k = 0; k < nf[2]; ++k)
+                    for (int j = 0; j < nf[1]; ++j)
+                        for (int i = 0; i < nf[0]; ++i) {
                             const size_t dst = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
                             const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
                             for (int a = 0; a < 3; ++a) bf_aos[dst + a] = f[a * c->nface + f0];

@@ -313,8 +313,10 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
             const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
+                // a thin (one-cell) axis has one face per cell, both sides
+                // of it (mhd/eos/adiabatic.h:164-177)
                 const double b_l = BF[(size_t)a * m.nface + f0];
-                const double b_r = BF[(size_t)a * m.nface + f0 + sh[a]];
+                const double b_r = m.n[a] < 2 ? b_l : BF[(size_t)a * m.nface + f0 + sh[a]];
                 const double bc_m = (b_l + b_r) / 2;
                 bc[a] = bc_m;
                 const double bcs2 = bc_m * bc_m;
@@ -384,7 +386,10 @@ __global__ void __launch_bounds__(kThreads) k_flux(MeshView<FT> m, int step) {
     const bool in1 = p[a1] >= 0 && p[a1] < m.n[a1];
     const bool in2 = p[a2] >= 0 && p[a2] < m.n[a2];
     if constexpr (MHD) {
+        // transverse ghost lines exist only along active axes
+        // (mhd/integrator/integrator.h:152-155)
         if (!in1 && !in2) return;
+        if ((!in1 && m.n[a1] < 2) || (!in2 && m.n[a2] < 2)) return;
     } else {
         if (!in1 || !in2) return;
     }
@@ -440,8 +445,11 @@ __device__ __forceinline__ FT flux_at(const MeshView<FT> &m, int b, int a, const
 }
 
 // Edge EMF with the corner-upwind correction, mhd/integrator/integrator.h
-// :189-327 (3-D, w_e = 0.25).  Cell-centred E outside the block is B x v
-// from the neighbour's primitives (:201-213); inside it equals the c2p ec.
+// :189-327.  Cell-centred E outside the block is B x v from the neighbour's
+// primitives (:201-213); inside it equals the c2p ec.  Thin (one-cell) axes
+// come last (m.ndim counts the active ones): they carry one edge layer, no
+// shift, no flux and no corner upwinding, and an edge next to one takes the
+// surviving sweep at weight 0.5 (:221-300).
 template <class FT>
 __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
     if (m.ts[m.geo[row_block(m)].grp].skip) return;
@@ -452,37 +460,46 @@ __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
     i[0] = (int)(t % (m.n[0] + 1));
     i[1] = (int)((t / (m.n[0] + 1)) % (m.n[1] + 1));
     i[2] = (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)));
+    const int n_dm = m.ndim;
+    for (int a = n_dm; a < 3; ++a)
+        if (i[a] > 0) return;
     for (int ax = 0; ax < 3; ++ax) {
         if (i[ax] >= m.n[ax]) continue;
         const int ax_1 = mod3(1 + ax), ax_2 = mod3(2 + ax);
+        const bool th_1 = ax_1 >= n_dm, th_2 = ax_2 >= n_dm;
         const int axis[3] = {ax, ax_1, ax_2};
         FT ec[2][2];
         ec[1][1] = ec_at(m, b, i, axis);
-        --i[ax_1];
+        if (!th_1) --i[ax_1];
         ec[1][0] = ec_at(m, b, i, axis);
-        --i[ax_2];
+        if (!th_2) --i[ax_2];
         ec[0][0] = ec_at(m, b, i, axis);
-        ++i[ax_1];
+        if (!th_1) ++i[ax_1];
         ec[0][1] = ec_at(m, b, i, axis);
-        ++i[ax_2];
+        if (!th_2) ++i[ax_2];
         FT ee = 0;
-        const FT w_e = (FT)0.25f;
+        const FT w_e = (th_1 || th_2) ? (FT)0.5f : (FT)0.25f;
 #pragma unroll
         for (int pi = 0; pi < 2; ++pi) {
             const int a_para = pi == 0 ? ax_1 : ax_2;
             const int a_perp = pi == 0 ? ax_2 : ax_1;
-            --i[a_perp];
-            const FT ef_m = flux_at(m, b, a_para, i, BCC + ax);
-            ++i[a_perp];
-            const FT ef_p = flux_at(m, b, a_para, i, BCC + ax);
-            ee += w_e * (ef_m + ef_p);
-            const FT frho_p = flux_at(m, b, a_perp, i, RHO);
-            --i[a_para];
-            const FT frho_m = flux_at(m, b, a_perp, i, RHO);
-            ++i[a_para];
-            const FT de_p = (frho_p > 0 ? (ec[0][1] - ef_m) : (ec[1][1] - ef_p));
-            const FT de_m = (frho_m > 0 ? -(ec[0][0] - ef_m) : -(ec[1][0] - ef_p));
-            ee += w_e * (de_m - de_p);
+            const bool thn_p = a_perp >= n_dm;
+            if (a_para < n_dm) {
+                if (!thn_p) --i[a_perp];
+                const FT ef_m = flux_at(m, b, a_para, i, BCC + ax);
+                if (!thn_p) ++i[a_perp];
+                const FT ef_p = flux_at(m, b, a_para, i, BCC + ax);
+                ee += w_e * (ef_m + ef_p);
+                if (!thn_p) {
+                    const FT frho_p = flux_at(m, b, a_perp, i, RHO);
+                    --i[a_para];
+                    const FT frho_m = flux_at(m, b, a_perp, i, RHO);
+                    ++i[a_para];
+                    const FT de_p = (frho_p > 0 ? (ec[0][1] - ef_m) : (ec[1][1] - ef_p));
+                    const FT de_m = (frho_m > 0 ? -(ec[0][0] - ef_m) : -(ec[1][0] - ef_p));
+                    ee += w_e * (de_m - de_p);
+                }
+            }
             if (pi == 0) {
                 const FT swp = ec[0][1];
                 ec[0][1] = ec[1][0];
@@ -585,6 +602,10 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
         for (int v = 0; v < 5; ++v) Ut[(size_t)v * m.ncell] = uu[v];
     }
     if constexpr (MHD) {
+        // the index-1 slot of a thin axis is not a face of the block: it
+        // is never computed, and every reader takes slot 0 for both sides
+        for (int a = m.ndim; a < 3; ++a)
+            if (i[a] > 0) return;
         const FT *EE = m.EE + (size_t)b * 3 * m.nface;
         double *Bs = m.BF + ((size_t)b * 2 + step) * 3 * m.nface + t;
         double *Bt = m.BF + ((size_t)b * 2 + (1 - step)) * 3 * m.nface + t;
@@ -593,13 +614,19 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
         for (int a = 0; a < 3; ++a) {
             const int ax_1 = mod3(a + 1), ax_2 = mod3(a + 2);
             if (i[ax_1] >= m.n[ax_1] || i[ax_2] >= m.n[ax_2]) continue;
+            // an edge pair spanning a thin axis vanishes
+            // (mhd/integrator/integrator.h:419-447)
+            const bool th_1 = ax_1 >= m.ndim, th_2 = ax_2 >= m.ndim;
             // FP64 edge circulation (also in the mixed form: the EMF is
             // widened before the product so CT keeps div B at round-off).
-            double db = (double)EE[(size_t)ax_1 * m.nface + t] * g.dx[ax_1];
-            db -= (double)EE[(size_t)ax_2 * m.nface + t] * g.dx[ax_2];
-            db += (double)EE[(size_t)ax_2 * m.nface + t + sh[ax_1]] * g.dx[ax_2];
-            db -= (double)EE[(size_t)ax_1 * m.nface + t + sh[ax_2]] * g.dx[ax_1];
-            const double sfc = g.sfc[a];
+            double db = th_2 ? 0.0 : (double)EE[(size_t)ax_1 * m.nface + t] * g.dx[ax_1];
+            if (!th_1) {
+                db -= (double)EE[(size_t)ax_2 * m.nface + t] * g.dx[ax_2];
+                db += (double)EE[(size_t)ax_2 * m.nface + t + sh[ax_1]] * g.dx[ax_2];
+            }
+            if (!th_2) db -= (double)EE[(size_t)ax_1 * m.nface + t + sh[ax_2]] * g.dx[ax_1];
+            // the thin face family's area (sfc_thin, integrator.h:74-81)
+            const double sfc = a >= m.ndim ? g.dx[ax_1] * g.dx[ax_2] : g.sfc[a];
             db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
             const double bs = Bs[(size_t)a * m.nface];
             Bt[(size_t)a * m.nface] =
@@ -623,7 +650,7 @@ __global__ void __launch_bounds__(kThreads) k_divb(MeshView<FT> m, double *part)
         const size_t f0 = fidx(m.n, i, j, k);
         const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
         double s = 0;
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < m.ndim; ++a)      // no variation along a thin axis
             s += (BF[(size_t)a * m.nface + f0 + sh[a]] - BF[(size_t)a * m.nface + f0]) / m.geo[b].dx[a];
         v = kfabs(s);
     }

@@ -30,7 +30,8 @@ dual_t::dual_t(block_t &b, kr_ctx *ctx, bool mhd)
     d.hyd.n_var = mhd ? 8 : 5;
     d.hyd.u_data.assign((size_t)b.geo.n_ceff.prod() * d.hyd.n_var, 0.0);
     if (mhd) {
-        for (int a = 0; a < 3; ++a) d.fld.N[a] = b.geo.n_ceff[a] + 5;
+        // n + (n > 1) faces plus two ghost layers each side (fld_data.cpp:18-26)
+        for (int a = 0; a < 3; ++a) d.fld.N[a] = b.geo.n_ceff[a] + (b.geo.n_ceff[a] > 1) + 4;
         d.fld.bf_data.assign((size_t)d.fld.N.prod() * 3, 0.0);
     }
 }

@@ -19,6 +19,23 @@ import numpy as np
 from . import _capi
 
 
+def bf_faces(n):
+    """Face indices per axis of the face-field array: n + 1 on an active
+    axis, 1 on a thin (one-cell) axis (mhd/types/fld_data.cpp:18-26)."""
+    return tuple(int(v) + (1 if v > 1 else 0) for v in n)
+
+
+def bf_host_dims(n):
+    """(N0, N1, N2) of a block's host face-field array: the faces plus two
+    ghost layers on each side of every axis."""
+    return tuple(f + 4 for f in bf_faces(n))
+
+
+def bf_host_size(n):
+    N = bf_host_dims(n)
+    return N[0] * N[1] * N[2] * 3
+
+
 @dataclass
 class MeshSpec:
     n_cell_global: tuple
@@ -139,20 +156,19 @@ class MeshSpec:
             u_aos = np.ascontiguousarray(np.moveaxis(ub, 0, -1)).reshape(-1)
             bf_aos = None
             if mhd:
-                bf_aos = np.zeros((n[2] + 5, n[1] + 5, n[0] + 5, 3))
+                nf = bf_faces(n)
+                bf_aos = np.zeros(bf_host_dims(n)[::-1] + (3,))
                 r = self.reg(b)
                 for c, g in enumerate(bfs):
-                    zi = (np.arange(n[2] + 1) + r[2] * n[2])
-                    yi = (np.arange(n[1] + 1) + r[1] * n[1])
-                    xi = (np.arange(n[0] + 1) + r[0] * n[0])
-                    # transverse wrap (periodic); the component axis keeps N+1 faces
-                    if c != 2:
-                        zi = zi % self.n_cell_global[2]
-                    if c != 1:
-                        yi = yi % self.n_cell_global[1]
-                    if c != 0:
-                        xi = xi % self.n_cell_global[0]
-                    bf_aos[2:n[2] + 3, 2:n[1] + 3, 2:n[0] + 3, c] = g[np.ix_(zi, yi, xi)]
+                    # nf[a] face indices per axis: n + 1 on an active axis (the
+                    # transverse +1 slot wraps periodically), 1 on a thin axis
+                    idx = []
+                    for a in range(3):
+                        ii = np.arange(nf[a]) + r[a] * n[a]
+                        if a != c:
+                            ii = ii % self.n_cell_global[a]
+                        idx.append(ii)
+                    bf_aos[2:nf[2] + 2, 2:nf[1] + 2, 2:nf[0] + 2, c] = g[np.ix_(idx[2], idx[1], idx[0])]
                 bf_aos = bf_aos.reshape(-1)
             out.append((u_aos, bf_aos))
         return out
@@ -172,10 +188,20 @@ class MeshSpec:
             ub = np.moveaxis(np.asarray(u_aos).reshape(n[2], n[1], n[0], nvar), -1, 0)
             u[:, sz, sy, sx] = ub[:5]
             if mhd:
-                f = np.asarray(bf_aos).reshape(n[2] + 5, n[1] + 5, n[0] + 5, 3)
-                bfs[0][sz, sy, sx.start:sx.stop + 1] = f[2:n[2] + 2, 2:n[1] + 2, 2:n[0] + 3, 0]
-                bfs[1][sz, sy.start:sy.stop + 1, sx] = f[2:n[2] + 2, 2:n[1] + 3, 2:n[0] + 2, 1]
-                bfs[2][sz.start:sz.stop + 1, sy, sx] = f[2:n[2] + 3, 2:n[1] + 2, 2:n[0] + 2, 2]
+                f = np.asarray(bf_aos).reshape(bf_host_dims(n)[::-1] + (3,))
+                for c in range(3):
+                    # component c: n + 1 faces along its axis (on a thin axis
+                    # the one face is both sides of the cell), n across
+                    sl = [slice(2, n[a] + 2) for a in range(3)]
+                    dst = list(self.block_slices(b))
+                    if n[c] > 1:
+                        sl[c] = slice(2, n[c] + 3)
+                        dst[c] = slice(dst[c].start, dst[c].stop + 1)
+                    v = f[sl[2], sl[1], sl[0], c]
+                    bfs[c][dst[2], dst[1], dst[0]] = v
+                    if n[c] == 1:
+                        dst[c] = slice(dst[c].start + 1, dst[c].stop + 1)
+                        bfs[c][dst[2], dst[1], dst[0]] = v
         return u, bfs
 
 
@@ -434,7 +460,7 @@ class Context:
         blocks = []
         for b in range(self.spec.n_blocks):
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
-            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            ba = np.zeros(bf_host_size(n)) if self.mhd else None
             _capi.check(self.lib.kr_download(self.h, off + b, ua.ctypes.data,
                                              ba.ctypes.data if ba is not None else None),
                         self.h, "kr_download")
@@ -458,7 +484,7 @@ class Context:
         out = []
         for b in range(self.spec.n_blocks):
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
-            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            ba = np.zeros(bf_host_size(n)) if self.mhd else None
             _capi.check(self.lib.kr_download(self.h, b, ua.ctypes.data,
                                              ba.ctypes.data if ba is not None else None),
                         self.h, "kr_download")

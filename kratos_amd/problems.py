@@ -10,6 +10,7 @@ same seed gives the same state on every machine.
   a seeded edge vector potential (div B = 0 to round-off), rescaled to the
   requested rms Mach number and plasma beta (configs 2, 3, 5).
 * ``blast``      -- MHD blast wave in a uniform oblique field (config 4).
+* ``orszag_tang``, ``brio_wu`` -- the 2-D and 1-D MHD classics.
 """
 import math
 
@@ -175,6 +176,52 @@ def blast(N, gamma=5.0 / 3.0, rho0=1.0, p0=0.1, p_in=10.0, r_in=0.1,
     bx = np.full((Nz, Ny, Nx + 1), b0[0])
     by = np.full((Nz, Ny + 1, Nx), b0[1])
     bz = np.full((Nz + 1, Ny, Nx), b0[2])
+    u = np.zeros((5, Nz, Ny, Nx))
+    u[0] = rho
+    u[1] = total_energy(rho, pre, [0 * rho] * 3, cell_centred_b(bx, by, bz), gamma)
+    return u, (bx, by, bz)
+
+
+def orszag_tang(N=(256, 256, 1), gamma=5.0 / 3.0):
+    """Orszag-Tang vortex (Orszag & Tang 1979, in the normalisation of
+    Stone et al. 2008, ApJS 178, sec. 5.5) on the periodic unit square:
+    rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2 pi y, sin 2 pi x, 0) and
+    B = B0 (-sin 2 pi y, sin 4 pi x, 0) with B0 = 1/sqrt(4 pi).  bx is
+    sampled at the x-faces (a function of y only) and by at the y-faces (of
+    x only), so the discrete div B is exactly zero.  N[2] = 1 gives the 2-D
+    problem (bz has its two coincident z-faces)."""
+    Nx, Ny, Nz = N
+    xc = (np.arange(Nx) + 0.5) / Nx
+    yc = (np.arange(Ny) + 0.5) / Ny
+    b0 = 1.0 / math.sqrt(4.0 * math.pi)
+    rho = np.full((Nz, Ny, Nx), 25.0 / (36.0 * math.pi))
+    pre = np.full((Nz, Ny, Nx), 5.0 / (12.0 * math.pi))
+    vx = np.broadcast_to(-np.sin(2 * math.pi * yc)[None, :, None], (Nz, Ny, Nx))
+    vy = np.broadcast_to(np.sin(2 * math.pi * xc)[None, None, :], (Nz, Ny, Nx))
+    vel = [vx, vy, 0 * rho]
+    bx = np.broadcast_to((-b0 * np.sin(2 * math.pi * yc))[None, :, None], (Nz, Ny, Nx + 1)).copy()
+    by = np.broadcast_to((b0 * np.sin(4 * math.pi * xc))[None, None, :], (Nz, Ny + 1, Nx)).copy()
+    bz = np.zeros((Nz + 1, Ny, Nx))
+    u = np.zeros((5, Nz, Ny, Nx))
+    u[0] = rho
+    for a in range(3):
+        u[2 + a] = rho * vel[a]
+    u[1] = total_energy(rho, pre, vel, cell_centred_b(bx, by, bz), gamma)
+    return u, (bx, by, bz)
+
+
+def brio_wu(N=(512, 1, 1), gamma=2.0, x_split=0.5, bx0=0.75):
+    """Brio & Wu (1988) MHD shock tube along x on [0, 1]: (rho, p, By) =
+    (1, 1, 1) for x < x_split and (0.125, 0.1, -1) beyond, Bx = 0.75,
+    gamma = 2.  N = (n, 1, 1) is the 1-D problem."""
+    Nx, Ny, Nz = N
+    xc = (np.arange(Nx) + 0.5) / Nx
+    left = xc < x_split
+    rho = np.broadcast_to(np.where(left, 1.0, 0.125)[None, None, :], (Nz, Ny, Nx)).copy()
+    pre = np.broadcast_to(np.where(left, 1.0, 0.1)[None, None, :], (Nz, Ny, Nx))
+    bx = np.full((Nz, Ny, Nx + 1), bx0)
+    by = np.broadcast_to(np.where(left, 1.0, -1.0)[None, None, :], (Nz, Ny + 1, Nx)).copy()
+    bz = np.zeros((Nz + 1, Ny, Nx))
     u = np.zeros((5, Nz, Ny, Nx))
     u[0] = rho
     u[1] = total_energy(rho, pre, [0 * rho] * 3, cell_centred_b(bx, by, bz), gamma)

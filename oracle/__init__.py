@@ -41,6 +41,16 @@ class KroConfig(C.Structure):
 _lib = None
 
 
+def _bf_size(n):
+    """Doubles in a block's face-field array: n + (n > 1) faces plus two
+    ghost layers each side, per axis, times 3 components
+    (mhd/types/fld_data.cpp:18-26)."""
+    s = 3
+    for v in n:
+        s *= v + (1 if v > 1 else 0) + 4
+    return s
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -153,7 +163,7 @@ class OracleMesh:
         blocks = []
         for b in range(self.spec.n_blocks):
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
-            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            ba = np.zeros(_bf_size(n)) if self.mhd else None
             self.lib.kro_get_block(self.h, b, ua.ctypes.data,
                                    ba.ctypes.data if ba is not None else None)
             blocks.append((ua, ba))
@@ -175,7 +185,7 @@ class OracleMesh:
         out = []
         for b in range(self.spec.n_blocks):
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
-            ba = np.zeros((n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3) if self.mhd else None
+            ba = np.zeros(_bf_size(n)) if self.mhd else None
             self.lib.kro_get_block(self.h, b, ua.ctypes.data,
                                    ba.ctypes.data if ba is not None else None)
             out.append((ua, ba))

@@ -509,10 +509,18 @@ struct Mesh {
                 throw std::runtime_error("Mesh size indivisible by sub-mesh.");
             nblk[a] = cfg.ng[a] / cfg.nb[a];
         }
-        if (cfg.mhd)
-            for (int a = 0; a < 3; ++a)
-                if (cfg.nb[a] < 2)
-                    throw std::runtime_error("oracle MHD needs 3-D blocks");
+        // MHD thin axes (one cell, mhd/integrator/integrator.h:221-227 "kept
+        // last by convention") must be the trailing ones and one cell wide
+        // over the whole mesh: they have no neighbour (keeper.h:59-99)
+        if (cfg.mhd) {
+            if (cfg.nb[0] < 2) throw std::runtime_error("MHD needs at least two cells along x");
+            for (int a = 1; a < 3; ++a) {
+                if (cfg.nb[a] < 2 && cfg.ng[a] != 1)
+                    throw std::runtime_error("an MHD thin axis must be one cell wide");
+                if (cfg.nb[a] >= 2 && cfg.nb[a - 1] < 2)
+                    throw std::runtime_error("MHD thin axes must be the trailing ones");
+            }
+        }
         if (!levels) {
             blocks.resize((size_t)nblk[0] * nblk[1] * nblk[2]);
             for (int bz = 0; bz < nblk[2]; ++bz)
@@ -981,9 +989,12 @@ struct Mesh {
                     double bc_2[3] = {0, 0, 0}, b_max[3] = {0, 0, 0}, bc[3];
                     double e_m(0);
                     for (int a = 0; a < 3; ++a) {
+                        // mhd/eos/adiabatic.h:164-177: a thin axis has one
+                        // face per cell, both sides of the cell
+                        const bool thin = b.n[a] < 2;
                         const double b_l = bf[a];
                         ++ii[a];
-                        const double b_r = b.bf.at(ii, step)[a];
+                        const double b_r = thin ? b_l : b.bf.at(ii, step)[a];
                         --ii[a];
                         const double bc_m = (b_l + b_r) / 2;
                         bc[a] = bc_m;
@@ -1054,11 +1065,11 @@ struct Mesh {
                     for (int n = 0; n < nv; ++n) d[n] = s[n];
                 }
         if (!cfg.mhd) return;
-        for (int ax = 0; ax < 3; ++ax)
+        for (int ax = 0; ax < b.ndim; ++ax)       // a thin axis has no neighbour
             for (int side = 0; side < 2; ++side) {
                 const int a1 = mod3(ax + 1), a2 = mod3(ax + 2);
-                for (int t2 = 0; t2 <= b.n[a2]; ++t2)
-                    for (int t1 = 0; t1 <= b.n[a1]; ++t1) {
+                for (int t2 = 0; t2 < b.bf.N[a2] - 4; ++t2)
+                    for (int t1 = 0; t1 < b.bf.N[a1] - 4; ++t1) {
                         int loc[3];
                         loc[ax] = side ? b.n[ax] : -1;
                         loc[a1] = t1;
@@ -1151,17 +1162,22 @@ struct Mesh {
         for (int side = 0; side < 6; ++side) {
             const int axis = side >> 1;
             const bool right = side & 1;
-            if (!b.phy[axis][right]) continue;
+            // mesh/boundary/keeper.cpp:100-103: physical sides of the
+            // active axes only
+            if (axis >= b.ndim || !b.phy[axis][right]) continue;
             const int kind = cfg.bnd[side];
             const int ap1 = mod3(axis + 1), ap2 = mod3(axis + 2);
             const int n_ceff = b.n[axis];
+            // a thin transverse axis has no ghost band (mhd/boundary/phys.h
+            // :80-90, :154-157)
+            const int g1 = b.n[ap1] > 1 ? n_gh : 0, g2 = b.n[ap2] > 1 ? n_gh : 0;
             auto f_i = [&](int a, int v, int i_m) {
                 if (b.phy[a][0]) v = std::max(v, 0);
                 if (b.phy[a][1]) v = std::min(v, i_m);
                 return v;
             };
-            for (int t2 = -n_gh; t2 < b.n[ap2] + n_gh; ++t2)
-                for (int t1 = -n_gh; t1 < b.n[ap1] + n_gh; ++t1) {
+            for (int t2 = -g2; t2 < b.n[ap2] + g2; ++t2)
+                for (int t1 = -g1; t1 < b.n[ap1] + g1; ++t1) {
                     int i_tgt[3], i_src[3], i_sfc[3];
                     i_tgt[ap1] = t1;
                     i_tgt[ap2] = t2;
@@ -1352,10 +1368,15 @@ struct Mesh {
             const double *v = b.w.at(j) + MOM;
             return bb[ax[1]] * v[ax[2]] - bb[ax[2]] * v[ax[1]];
         };
+        const int n_dm = b.ndim;
         for (int i2 = 0; i2 <= b.n[2]; ++i2)
             for (int i1 = 0; i1 <= b.n[1]; ++i1)
                 for (int i0 = 0; i0 <= b.n[0]; ++i0) {
                     int i[3] = {i0, i1, i2};
+                    // thin axes carry one edge layer (integrator.h:221-230)
+                    bool th_skip = false;
+                    for (int a = n_dm; a < 3; ++a) th_skip |= i[a] > 0;
+                    if (th_skip) continue;
                     double ee[3] = {0, 0, 0};
                     double ec[2][2];
                     int axis[3];
@@ -1364,34 +1385,44 @@ struct Mesh {
                         if (i[ax] >= b.n[ax]) continue;
                         ax_1 = mod3(1 + ax);
                         ax_2 = mod3(2 + ax);
+                        // no shift along a thin axis (:236-252)
+                        const bool th_1 = ax_1 >= n_dm, th_2 = ax_2 >= n_dm;
                         ec[1][1] = f_ec(i, axis);
-                        --i[ax_1];
+                        if (!th_1) --i[ax_1];
                         ec[1][0] = f_ec(i, axis);
-                        --i[ax_2];
+                        if (!th_2) --i[ax_2];
                         ec[0][0] = f_ec(i, axis);
-                        ++i[ax_1];
+                        if (!th_1) ++i[ax_1];
                         ec[0][1] = f_ec(i, axis);
-                        ++i[ax_2];
+                        if (!th_2) ++i[ax_2];
                         ee[ax] = 0;
-                        const double w_e = 0.25f;
+                        // the surviving sweep takes the full weight when a
+                        // thin axis is one of the two (:258-266)
+                        const double w_e = (th_1 || th_2) ? 0.5f : 0.25f;
                         const int paras[2] = {ax_1, ax_2};
                         for (int pi = 0; pi < 2; ++pi) {
                             const int a_para = paras[pi];
                             const int a_perp = (a_para == ax_1 ? ax_2 : ax_1);
-                            --i[a_perp];
-                            const double ef_m = b.f[a_para].at(i)[BCC + ax];
-                            ++i[a_perp];
-                            const double ef_p = b.f[a_para].at(i)[BCC + ax];
-                            ee[ax] += w_e * (ef_m + ef_p);
-                            const double frho_p = b.f[a_perp].at(i)[RHO];
-                            --i[a_para];
-                            const double frho_m = b.f[a_perp].at(i)[RHO];
-                            ++i[a_para];
-                            const double de_p = (frho_p > 0 ? (ec[0][1] - ef_m)
-                                                            : (ec[1][1] - ef_p));
-                            const double de_m = (frho_m > 0 ? -(ec[0][0] - ef_m)
-                                                            : -(ec[1][0] - ef_p));
-                            ee[ax] += w_e * (de_m - de_p);
+                            const bool thn_p = a_perp >= n_dm;
+                            // a thin sweep direction has no flux (:272-276)
+                            if (a_para < n_dm) {
+                                if (!thn_p) --i[a_perp];
+                                const double ef_m = b.f[a_para].at(i)[BCC + ax];
+                                if (!thn_p) ++i[a_perp];
+                                const double ef_p = b.f[a_para].at(i)[BCC + ax];
+                                ee[ax] += w_e * (ef_m + ef_p);
+                                if (!thn_p) {       // corner upwinding
+                                    const double frho_p = b.f[a_perp].at(i)[RHO];
+                                    --i[a_para];
+                                    const double frho_m = b.f[a_perp].at(i)[RHO];
+                                    ++i[a_para];
+                                    const double de_p = (frho_p > 0 ? (ec[0][1] - ef_m)
+                                                                    : (ec[1][1] - ef_p));
+                                    const double de_m = (frho_m > 0 ? -(ec[0][0] - ef_m)
+                                                                    : -(ec[1][0] - ef_p));
+                                    ee[ax] += w_e * (de_m - de_p);
+                                }
+                            }
                             if (a_para != ax_1) continue;
                             const double swp = ec[0][1];
                             ec[0][1] = ec[1][0];
@@ -1428,26 +1459,39 @@ struct Mesh {
     /* -------- CT update (mhd/integrator/integrator.h:385-462,
      *          mhd/integrator/rk2.h:191-205) --------------------------- */
     void intg_field_block(Block &b, int step, double dt) {
+        const int n_dm = b.ndim;
         for (int i2 = 0; i2 <= b.n[2]; ++i2)
             for (int i1 = 0; i1 <= b.n[1]; ++i1)
                 for (int i0 = 0; i0 <= b.n[0]; ++i0) {
                     int i[3] = {i0, i1, i2};
+                    // the wrap slot of a thin axis (index 1) is a copy of
+                    // slot 0 made by the thin-axis sync below; the reference
+                    // computes it from an edge layer it never fills and
+                    // then overwrites it (integrator.h:598-632)
+                    bool th_skip = false;
+                    for (int a = n_dm; a < 3; ++a) th_skip |= i[a] > 0;
+                    if (th_skip) continue;
                     double db_v[3] = {0, 0, 0};
                     unsigned msk = 0;
                     for (int a = 0; a < 3; ++a) {
                         const int ax_1 = mod3(a + 1), ax_2 = mod3(a + 2);
                         if (i[ax_1] >= b.n[ax_1] || i[ax_2] >= b.n[ax_2]) continue;
                         msk |= 1u << a;
+                        // pairs spanning a thin axis vanish (integrator.h:419-447)
+                        const bool th_1 = ax_1 >= n_dm, th_2 = ax_2 >= n_dm;
                         double &db = db_v[a];
-                        db = (double)(b.ee.at(i)[ax_1] * b.dx0[ax_1]);
-                        db -= (double)(b.ee.at(i)[ax_2] * b.dx0[ax_2]);
+                        if (!th_2) db = (double)(b.ee.at(i)[ax_1] * b.dx0[ax_1]);
+                        else db = (double)(0);
+                        if (!th_1) db -= (double)(b.ee.at(i)[ax_2] * b.dx0[ax_2]);
                         ++i[ax_1];
-                        db += (double)(b.ee.at(i)[ax_2] * b.dx0[ax_2]);
+                        if (!th_1) db += (double)(b.ee.at(i)[ax_2] * b.dx0[ax_2]);
                         --i[ax_1];
                         ++i[ax_2];
-                        db -= (double)(b.ee.at(i)[ax_1] * b.dx0[ax_1]);
+                        if (!th_2) db -= (double)(b.ee.at(i)[ax_1] * b.dx0[ax_1]);
                         --i[ax_2];
-                        const double sfc = b.sfc0[a];
+                        // the thin face family's area is dx_f(a1) * dx_f(a2)
+                        // (sfc_thin, integrator.h:74-81)
+                        const double sfc = a >= n_dm ? b.dx0[ax_1] * b.dx0[ax_2] : b.sfc0[a];
                         db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
                     }
                     double *b_src = b.bf.at(i, step);
@@ -1457,6 +1501,34 @@ struct Mesh {
                             b_tgt[a] = db_v[a] + (step == 0 ? b_src[a]
                                                             : (b_src[a] + b_tgt[a]) / 2);
                 }
+        thin_axis_sync(b);
+    }
+
+    /* -------- thin-axis face sync (mhd/integrator/integrator.h:520-560,
+     * launched after every CT update at :585-600): along a periodic
+     * one-cell axis the wrap slots -2..2 of slot 0 are the slot-0 face,
+     * over the other two axes' interior and ghost band.                  */
+    void thin_axis_sync(Block &b) {
+        for (int a = 0; a < 3; ++a) {
+            if (b.n[a] != 1 || b.phy[a][0] || b.phy[a][1]) continue;
+            const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+            const int g = 2;
+            for (int t2 = -g; t2 < b.n[a2] + g; ++t2)
+                for (int t1 = -g; t1 < b.n[a1] + g; ++t1) {
+                    int i[3];
+                    i[a] = 0;
+                    i[a1] = t1;
+                    i[a2] = t2;
+                    const double *d0 = b.bf.at(i);
+                    for (int k = -g; k <= g; ++k) {
+                        if (k == 0) continue;
+                        int j[3] = {i[0], i[1], i[2]};
+                        j[a] = k;
+                        double *d1 = b.bf.at(j);
+                        for (int c = 0; c < 3; ++c) d1[c] = d0[c];
+                    }
+                }
+        }
     }
 
     /* -------- one RK substep (dynamics/dynamics.h:60-100) ------------ */

@@ -9,7 +9,8 @@ initial state each generator wrote.
   Python builder, final state bit-exact with the oracle.
 * vortex: 3-D Orszag-Tang-type MHD, 32^3 as 8 blocks of 16^3; n_cycle_lim
   = 9 runs 10 cycles (the reference's loop rule); final state bit-exact
-  with the oracle, div B at roundoff.
+  with the oracle, div B at roundoff; the same generator on a 2-D mesh
+  (64 x 64 x 1) set in the override file.
 """
 import os
 import subprocess
@@ -20,6 +21,7 @@ import pytest
 import kratos_amd
 import oracle
 from kratos_amd import problems
+from kratos_amd.mesh import bf_host_size
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +47,7 @@ def run_pgen(name, par, tmp_path, extra=""):
 def split_dump(raw, spec, mhd):
     n = spec.n_cell_block
     nu = n[0] * n[1] * n[2] * (8 if mhd else 5)
-    nf = (n[0] + 5) * (n[1] + 5) * (n[2] + 5) * 3 if mhd else 0
+    nf = bf_host_size(n) if mhd else 0
     per = nu + nf
     assert raw.size == 2 * spec.n_blocks * per
     states = []
@@ -88,6 +90,27 @@ def test_vortex_generator(gpu_lib, tmp_path):
     np.testing.assert_array_equal(u1, ou)
     for x, y in zip(b1, ob):
         np.testing.assert_array_equal(x, y)
+    divb = float(out.split("max |div B| =")[1].split()[0])
+    assert divb < 1e-12
+
+
+def test_vortex_generator_2d(gpu_lib, tmp_path):
+    """The same generator on a 2-D mesh (thin z) set in the override file:
+    64 x 64 x 1 as 4 blocks of 32 x 32 x 1."""
+    par = os.path.join(HOST, "usr_example", "vortex", "vortex.par")
+    spec = kratos_amd.MeshSpec((64, 64, 1), (32, 32, 1))
+    raw, out = run_pgen("vortex", par, tmp_path,
+                        "[mesh]\nn_cell_global = 64 64 1\nn_cell_block = 32 32 1\n")
+    (u0, b0), (u1, b1) = split_dump(raw, spec, True)
+    phys = kratos_amd.Physics(mhd=True, gamma=1.6666666666666667, cfl=0.3)
+    ref = oracle.OracleMesh(spec, phys)
+    ref.upload(u0, b0)
+    assert ref.advance(10) == 10
+    ou, ob = ref.download()
+    np.testing.assert_array_equal(u1, ou)
+    for x, y in zip(b1, ob):
+        np.testing.assert_array_equal(x, y)
+    assert np.abs(u1[3] - u0[3]).max() > 1e-3
     divb = float(out.split("max |div B| =")[1].split()[0])
     assert divb < 1e-12
 

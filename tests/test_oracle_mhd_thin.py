@@ -1,0 +1,198 @@
+"""Pins the oracle's 2-D and 1-D (thin-axis) MHD: one-cell trailing axes
+with one face per cell (mhd/eos/adiabatic.h:164-177), no sweep, flux or
+ghost line along them, edge EMFs next to a thin axis from the surviving
+sweep at weight 0.5 without corner upwinding, thin edge pairs dropped from
+the CT circulation and the thin face family's area dx_f(a1) dx_f(a2)
+(mhd/integrator/integrator.h:189-300, 385-456), and the thin-axis wrap
+sync (:520-600).
+
+* 1-D: the whole Heun cycle equals an independent finite-volume
+  restatement in numpy (PLM, HLLD through the oracle's per-face functor,
+  which its own tests pin; the face field By, Bz updated as cell averages
+  with the fluxes F_By = -E_z and F_Bz = E_y; the CFL dt from the fast
+  speed) to 1e-12.  A wrong EMF slot, sign, face index or area fails by
+  orders of magnitude.
+* 2-D, uniform along y, equals the 1-D run (the y-fluxes cancel and the
+  corner-upwind terms telescope): pins E_y (weight 0.5 with thin z), E_z,
+  By and the thin face family Bz.
+* 2-D, uniform along x, equals the 1-D run with x and y swapped (ideal MHD
+  is mirror invariant with B as a polar vector): pins E_x and the CT of
+  Bx.
+* Orszag-Tang in 2-D: div B stays at round-off and mass, momentum, energy
+  and the mean face field are conserved.
+* Thin axes must be trailing and one cell wide over the mesh.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kratos_amd import problems
+from kratos_amd.mesh import MeshSpec, Physics
+
+G = 5.0 / 3.0
+TP = 2 * math.pi
+
+
+def _profile(x):
+    """Smooth periodic 1-D magnetised state along the coordinate x."""
+    rho = 1.0 + 0.2 * np.sin(TP * x)
+    pre = 1.0 + 0.1 * np.cos(TP * x)
+    v = (0.3 * np.sin(TP * x), 0.2 * np.cos(TP * x), -0.1 * np.sin(2 * TP * x))
+    bn = 0.7
+    bt = (0.3 * np.cos(TP * x) + 0.1, 0.2 * np.sin(TP * x))
+    return rho, pre, v, bn, bt
+
+
+def _state_1d(n):
+    xc = (np.arange(n) + 0.5) / n
+    rho, pre, v, bn, (by, bz) = _profile(xc)
+    u = np.zeros((5, 1, 1, n))
+    u[0, 0, 0] = rho
+    for a in range(3):
+        u[2 + a, 0, 0] = rho * v[a]
+    u[1, 0, 0] = pre / (G - 1) + 0.5 * rho * sum(c * c for c in v) + 0.5 * (bn * bn + by * by + bz * bz)
+    bfs = (np.full((1, 1, n + 1), bn), np.broadcast_to(by, (1, 2, n)).copy(),
+           np.broadcast_to(bz, (2, 1, n)).copy())
+    return u, bfs
+
+
+def _run(spec, u, bfs, cycles, threads=2):
+    o = oracle.OracleMesh(spec, Physics(mhd=True, gamma=G, cfl=0.3), n_threads=threads)
+    o.upload(u, bfs)
+    assert o.advance(cycles) == cycles
+    return o.download()
+
+
+def _fv_1d(u, bfs, cycles, cfl=0.3, dt_expand=2.0):
+    """Independent 1-D restatement: cell-average (finite-volume) update of
+    the hydro state and of By, Bz; Bx is constant."""
+    n = u.shape[-1]
+    dx = 1.0 / n
+    U = [u[:, 0, 0, :].copy(), np.zeros((5, n))]
+    bx = bfs[0][0, 0, 0]
+    B = [np.stack([bfs[1][0, 0], bfs[2][0, 0]]), np.zeros((2, n))]   # By, Bz per cell
+    p_dt = 3.4028234663852886e38 / 10
+    for _ in range(cycles):
+        dt = None
+        for s in (0, 1):
+            Us, Bs = U[s], B[s]
+            rho = Us[0]
+            m = Us[2:5]
+            bc = np.stack([np.full(n, bx), Bs[0], Bs[1]])
+            e_m = 0.5 * (bc ** 2).sum(0)
+            vel = m / rho
+            pre = (Us[1] - e_m - 0.5 * (m ** 2).sum(0) / rho) * (G - 1)
+            if s == 0:
+                # fast speed along x from the face-maximum normal field
+                a2 = G * pre / rho
+                b2 = (bx * bx + bc[1] ** 2 + bc[2] ** 2) / rho
+                cf = np.sqrt(0.5 * (a2 + b2 + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bx * bx / rho)))
+                dt = min(p_dt, (cfl * dx / (cf + np.abs(vel[0]))).min())
+                p_dt = dt * dt_expand
+            w = np.concatenate([rho[None], pre[None], vel, bc])          # (8, n)
+            wg = np.concatenate([w[:, -2:], w, w[:, :2]], axis=1)         # periodic ghosts
+            ql = np.zeros((8, n + 1))
+            qr = np.zeros((8, n + 1))
+            for f in range(n + 1):
+                q_ll, q_l, q_r, q_rr = (wg[:, f + k] for k in range(4))
+                for v in range(8):
+                    if v == 5:
+                        ql[v, f] = qr[v, f] = bx
+                        continue
+                    dql, dqc, dqr = q_l[v] - q_ll[v], q_r[v] - q_l[v], q_rr[v] - q_r[v]
+                    ql[v, f] = q_l[v] + dql * dqc / (dql + dqc) if dql * dqc > 0 else q_l[v]
+                    qr[v, f] = q_r[v] - dqr * dqc / (dqr + dqc) if dqr * dqc > 0 else q_r[v]
+            F = np.stack([oracle.face_flux(True, 0, ql[:, f], qr[:, f], G) for f in range(n + 1)], 1)
+            h = dt * (1.0 if s == 0 else 0.5)
+            t = 1 - s
+            base_u = Us if s == 0 else (Us + U[t]) * 0.5
+            base_b = Bs if s == 0 else (Bs + B[t]) / 2
+            U[t] = base_u + h / dx * (F[:5, :-1] - F[:5, 1:])
+            f_by, f_bz = -F[7], F[6]         # EMF slots: E_z = -F_By, E_y = F_Bz
+            B[t] = base_b - h / dx * np.stack([f_by[1:] - f_by[:-1], f_bz[1:] - f_bz[:-1]])
+    return U[0], B[0], bx
+
+
+@pytest.mark.parametrize("nb", [64, 16])
+def test_1d_matches_finite_volume_restatement(nb):
+    n = 64
+    u, bfs = _state_1d(n)
+    spec = MeshSpec((n, 1, 1), (nb, 1, 1))
+    uo, (bx, by, bz) = _run(spec, u, bfs, 6)
+    U, B, bx0 = _fv_1d(u, bfs, 6)
+    assert np.abs(uo[:, 0, 0, :] - U).max() < 1e-12
+    assert np.abs(by[0, 0] - B[0]).max() < 1e-12 and np.abs(by[0, 1] - B[0]).max() < 1e-12
+    assert np.abs(bz[0, 0] - B[1]).max() < 1e-12 and np.abs(bz[1, 0] - B[1]).max() < 1e-12
+    assert np.all(bx == bx0)
+    # the state moved: a restatement that does nothing would not pass
+    assert np.abs(U - u[:, 0, 0, :]).max() > 1e-3
+    assert np.abs(B[1] - bfs[2][0, 0]).max() > 1e-3
+
+
+def test_2d_uniform_in_y_equals_1d():
+    n, ny = 64, 8
+    u1, b1 = _state_1d(n)
+    r1, (bx1, by1, bz1) = _run(MeshSpec((n, 1, 1), (16, 1, 1)), u1, b1, 8)
+    u2 = np.broadcast_to(u1, (5, 1, ny, n)).copy()
+    b2 = (np.broadcast_to(b1[0], (1, ny, n + 1)).copy(),
+          np.broadcast_to(b1[1][:, :1], (1, ny + 1, n)).copy(),
+          np.broadcast_to(b1[2], (2, ny, n)).copy())
+    r2, (bx2, by2, bz2) = _run(MeshSpec((n, ny, 1), (16, 4, 1)), u2, b2, 8)
+    assert np.abs(r2 - r1).max() < 1e-12
+    assert np.abs(bx2 - bx1).max() < 1e-12
+    assert np.abs(by2 - by1[:, :1]).max() < 1e-12
+    assert np.abs(bz2 - bz1).max() < 1e-12
+    assert np.abs(bz1[0, 0] - b1[2][0, 0]).max() > 1e-3
+
+
+def test_2d_uniform_in_x_equals_swapped_1d():
+    n, nx = 64, 8
+    u1, b1 = _state_1d(n)
+    r1, (bx1, by1, bz1) = _run(MeshSpec((n, 1, 1), (16, 1, 1)), u1, b1, 8)
+    # swap x and y: momenta 0 <-> 1, bx <-> by
+    u2 = np.zeros((5, 1, n, nx))
+    for v, s in enumerate((0, 1, 3, 2, 4)):
+        u2[v] = u1[s, :, 0, :][:, :, None]
+    b2 = (np.broadcast_to(b1[1][0, 0][None, :, None], (1, n, nx + 1)).copy(),
+          np.broadcast_to(b1[0][0, 0][None, :, None], (1, n + 1, nx)).copy(),
+          np.broadcast_to(b1[2][:, 0, :][:, :, None], (2, n, nx)).copy())
+    r2, (bx2, by2, bz2) = _run(MeshSpec((nx, n, 1), (4, 16, 1)), u2, b2, 8)
+    for v, s in enumerate((0, 1, 3, 2, 4)):
+        assert np.abs(r2[v] - r1[s, :, 0, :][:, :, None]).max() < 1e-12
+    assert np.abs(bx2 - by1[0, 0][None, :, None]).max() < 1e-12
+    assert np.abs(by2 - bx1[0, 0][None, :, None]).max() < 1e-12
+    assert np.abs(bz2 - bz1[:, 0, :][:, :, None]).max() < 1e-12
+
+
+def test_orszag_tang_divb_and_conservation():
+    N = (64, 64, 1)
+    spec = MeshSpec(N, (32, 16, 1))
+    u, bfs = problems.orszag_tang(N)
+    # add a z component so the thin face family and E_x, E_y are exercised
+    xc = (np.arange(N[0]) + 0.5) / N[0]
+    yc = (np.arange(N[1]) + 0.5) / N[1]
+    vz = 0.1 * np.sin(TP * (xc[None, :] + yc[:, None]))
+    bz = 0.05 * np.cos(TP * (xc[None, :] - 2 * yc[:, None]))
+    u[4, 0] = u[0, 0] * vz
+    u[1, 0] += 0.5 * u[0, 0] * vz * vz + 0.5 * bz * bz
+    bfs = (bfs[0], bfs[1], np.stack([bz, bz]))
+    uo, (bx, by, bzo) = _run(spec, u, bfs, 20, threads=4)
+    div = (bx[:, :, 1:] - bx[:, :, :-1]) * N[0] + (by[:, 1:, :] - by[:, :-1, :]) * N[1]
+    assert np.abs(div).max() < 1e-11
+    for v in range(5):
+        assert abs(uo[v].sum() - u[v].sum()) < 1e-11 * max(1.0, np.abs(u[v]).sum())
+    assert abs(bx.sum() - bfs[0].sum()) < 1e-11 and abs(by.sum() - bfs[1].sum()) < 1e-11
+    assert abs(bzo[0].sum() - bz.sum()) < 1e-11 and np.all(bzo[0] == bzo[1])
+    assert np.abs(bzo[0] - bz).max() > 1e-3
+
+
+@pytest.mark.parametrize("ng,nb,msg", [
+    ((16, 16, 16), (1, 16, 16), "two cells along x"),
+    ((16, 1, 16), (16, 1, 16), "trailing"),
+    ((16, 16, 2), (16, 16, 1), "one cell wide"),
+])
+def test_thin_axis_layout_rejected(ng, nb, msg):
+    with pytest.raises(RuntimeError, match=msg):
+        oracle.OracleMesh(MeshSpec(ng, nb), Physics(mhd=True), n_threads=1)
