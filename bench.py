@@ -16,7 +16,9 @@ reported beside it.
 accuracy: the same recipe at beta = 1 and beta = 0.01, both forms run to
 t = 0.1 (t_lim top-up); relative L1 of rho, E and B of the mixed form
 against the FP64 form (which equals the CPU oracle bit for bit on every
-case the parity tests run), and max |div B|.
+case the parity tests run), and max |div B|.  At beta = 0.01 also the L1
+distance that a 1e-12 relative perturbation of the initial density reaches
+in the FP64 form alone: the problem's own sensitivity floor.
 
 smr_blast: the MHD blast on a three-level statically refined mesh (128^3
 root cells in 16^3 blocks, ~2,000 blocks), both forms on the staged kernels
@@ -111,26 +113,39 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
                  "tune_ms": [round(t, 4) for t in st["tune_ms"]]}
 
 
-def accuracy_leg(n, nb, beta, t_lim, path):
-    """Both forms to t_lim from one IC; mixed-vs-FP64 L1 and div B."""
+def accuracy_leg(n, nb, beta, t_lim, path, sensitivity=False):
+    """Both forms to t_lim from one IC; mixed-vs-FP64 L1 and div B.  With
+    `sensitivity`, also the FP64 form from the same IC with the density
+    multiplied by 1 + 1e-12 * N(0, 1) (numpy seed 1): its L1 distance to
+    the unperturbed FP64 run is the floor that round-off-level differences
+    reach on this problem by t_lim, whatever the precision."""
     import kratos_amd
     spec = kratos_amd.MeshSpec((n, n, n), (nb, nb, nb))
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3, t_lim=t_lim, top_up=True)
     u, b, _ = make_problem(n, beta=beta)
+    runs = [("fp64", "fp64", u), ("mixed", "mixed", u)]
+    if sensitivity:
+        up = u.copy()
+        up[0] *= 1 + 1e-12 * np.random.default_rng(1).standard_normal(up[0].shape)
+        runs.append(("fp64_perturbed", "fp64", up))
     out = {}
-    for form in ("fp64", "mixed"):
+    for name, form, u0 in runs:
         ctx = kratos_amd.Context(spec, phys, form, path=path)
-        ctx.upload(u, b)
+        ctx.upload(u0, b)
         done = ctx.advance(1000000)
-        out[form] = (ctx.download(), ctx.divb_max(), ctx.time()[0], done)
+        out[name] = (ctx.download(), ctx.divb_max(), ctx.time()[0], done)
         ctx.close()
     (um, bm), divb_m, tm, cm = out["mixed"]
     (ud, bd), divb_d, td, cd = out["fp64"]
     cat = lambda bs: np.concatenate([x.ravel() for x in bs])
-    return {"beta": beta, "t": tm, "cycles_mixed": cm, "cycles_fp64": cd,
-            "l1_rho": rel_l1(um[0], ud[0]), "l1_E": rel_l1(um[1], ud[1]),
-            "l1_B": rel_l1(cat(bm), cat(bd)),
-            "divb_max_mixed": divb_m, "divb_max_fp64": divb_d}
+    res = {"beta": beta, "t": tm, "cycles_mixed": cm, "cycles_fp64": cd,
+           "l1_rho": rel_l1(um[0], ud[0]), "l1_E": rel_l1(um[1], ud[1]),
+           "l1_B": rel_l1(cat(bm), cat(bd)),
+           "divb_max_mixed": divb_m, "divb_max_fp64": divb_d}
+    if sensitivity:
+        (up_, _), _, _, _ = out["fp64_perturbed"]
+        res["l1_rho_fp64_perturbed_1e-12"] = rel_l1(up_[0], ud[0])
+    return res
 
 
 def ensemble_leg(n_box, n, t_lim, path):
@@ -376,7 +391,7 @@ def main():
     acc = []
     if not args.no_accuracy:
         for beta in (1.0, 0.01):
-            a = accuracy_leg(n, nb, beta, args.t_acc, args.path)
+            a = accuracy_leg(n, nb, beta, args.t_acc, args.path, sensitivity=beta < 0.1)
             acc.append(a)
             print(f"[bench] accuracy beta={beta}: {a}", file=sys.stderr, flush=True)
     ens = None
