@@ -18,8 +18,9 @@
  *           :18-26: a thin, one-cell axis has a single face, both sides
  *           of the cell), idx = (((k+2)*N1 + (j+2))*N0 + (i+2))*3 + c
  *
- * MHD blocks may be 2-D or 1-D: the one-cell (thin) axes must be the
- * trailing ones (z, or y and z) and one cell wide over the whole mesh
+ * Blocks may be 2-D or 1-D: the one-cell (thin) axes must be the trailing
+ * ones (z, or y and z; the sweeps run over the first n_dim axes,
+ * geometry.cpp:46), and for MHD one cell wide over the whole mesh
  * (mhd/integrator/integrator.h:221-227, 419-456).
  */
 #ifndef KRATOS_AMD_H

@@ -855,12 +855,15 @@ int kr_create(const kr_config *cfg, kr_ctx **out) {
     else if (cfg->precision != KR_FP64 && cfg->precision != KR_MIXED) why = "unknown precision";
     else if (!(cfg->gamma > 1)) why = "gamma must exceed 1";
     else if (!(cfg->cfl > 0)) why = "cfl must be positive";
-    // MHD thin (one-cell) axes are the trailing ones, as the reference's
-    // EMF and CT assume (mhd/integrator/integrator.h:221-227)
-    if (c->mhd && why.empty()) {
-        if (cfg->n_cell_block[0] < 2) why = "MHD needs at least two cells along x";
-        else if (cfg->n_cell_block[1] < 2 && cfg->n_cell_block[2] >= 2)
-            why = "MHD thin axes must be the trailing ones";
+    // thin (one-cell) axes are the trailing ones: the sweeps run over the
+    // first n_dim axes, n_dim counting the axes with more than one cell
+    // (geometry.cpp:46, dynamics calc_flx), and the MHD EMF and CT assume
+    // the same (mhd/integrator/integrator.h:221-227)
+    if (why.empty()) {
+        if (c->mhd && cfg->n_cell_block[0] < 2) why = "MHD needs at least two cells along x";
+        else if ((cfg->n_cell_block[0] < 2 && (cfg->n_cell_block[1] >= 2 || cfg->n_cell_block[2] >= 2)) ||
+                 (cfg->n_cell_block[1] < 2 && cfg->n_cell_block[2] >= 2))
+            why = "thin axes must be the trailing ones";
     }
     for (int a = 0; a < 3 && why.empty(); ++a) {
         if (cfg->n_cell_block[a] < 1) why = "n_cell_block must be >= 1";

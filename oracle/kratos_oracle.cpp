@@ -512,14 +512,16 @@ struct Mesh {
         // MHD thin axes (one cell, mhd/integrator/integrator.h:221-227 "kept
         // last by convention") must be the trailing ones and one cell wide
         // over the whole mesh: they have no neighbour (keeper.h:59-99)
+        // (the sweeps of both models run over the first b.ndim axes, which
+        // counts the axes with more than one cell, geometry.cpp:46)
+        for (int a = 1; a < 3; ++a)
+            if (cfg.nb[a] >= 2 && cfg.nb[a - 1] < 2)
+                throw std::runtime_error("thin axes must be the trailing ones");
         if (cfg.mhd) {
             if (cfg.nb[0] < 2) throw std::runtime_error("MHD needs at least two cells along x");
-            for (int a = 1; a < 3; ++a) {
+            for (int a = 1; a < 3; ++a)
                 if (cfg.nb[a] < 2 && cfg.ng[a] != 1)
                     throw std::runtime_error("an MHD thin axis must be one cell wide");
-                if (cfg.nb[a] >= 2 && cfg.nb[a - 1] < 2)
-                    throw std::runtime_error("MHD thin axes must be the trailing ones");
-            }
         }
         if (!levels) {
             blocks.resize((size_t)nblk[0] * nblk[1] * nblk[2]);

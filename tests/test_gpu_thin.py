@@ -129,6 +129,9 @@ def test_thin_layout_rejected(gpu_lib):
     phys = kratos_amd.Physics(mhd=True)
     with pytest.raises(RuntimeError):
         kratos_amd.Context(kratos_amd.MeshSpec((16, 1, 16), (16, 1, 16)), phys)
+    with pytest.raises(RuntimeError):
+        kratos_amd.Context(kratos_amd.MeshSpec((16, 1, 16), (16, 1, 16)),
+                           kratos_amd.Physics(mhd=False))
     u, b = problems.orszag_tang((16, 16, 2))
     ctx = kratos_amd.Context(kratos_amd.MeshSpec((16, 16, 2), (16, 16, 1)), phys)
     with pytest.raises(RuntimeError, match="thin axis"):

@@ -20,7 +20,8 @@ sync (:520-600).
   Bx.
 * Orszag-Tang in 2-D: div B stays at round-off and mass, momentum, energy
   and the mean face field are conserved.
-* Thin axes must be trailing and one cell wide over the mesh.
+* Thin axes must be trailing (hydro too: the sweeps run over the first
+  n_dim axes) and, for MHD, one cell wide over the mesh.
 """
 import math
 
@@ -188,11 +189,13 @@ def test_orszag_tang_divb_and_conservation():
     assert np.abs(bzo[0] - bz).max() > 1e-3
 
 
-@pytest.mark.parametrize("ng,nb,msg", [
-    ((16, 16, 16), (1, 16, 16), "two cells along x"),
-    ((16, 1, 16), (16, 1, 16), "trailing"),
-    ((16, 16, 2), (16, 16, 1), "one cell wide"),
+@pytest.mark.parametrize("mhd,ng,nb,msg", [
+    (True, (1, 1, 1), (1, 1, 1), "two cells along x"),
+    (True, (16, 16, 16), (1, 16, 16), "trailing"),
+    (True, (16, 1, 16), (16, 1, 16), "trailing"),
+    (False, (16, 1, 16), (16, 1, 16), "trailing"),
+    (True, (16, 16, 2), (16, 16, 1), "one cell wide"),
 ])
-def test_thin_axis_layout_rejected(ng, nb, msg):
+def test_thin_axis_layout_rejected(mhd, ng, nb, msg):
     with pytest.raises(RuntimeError, match=msg):
-        oracle.OracleMesh(MeshSpec(ng, nb), Physics(mhd=True), n_threads=1)
+        oracle.OracleMesh(MeshSpec(ng, nb), Physics(mhd=mhd), n_threads=1)
