@@ -182,6 +182,13 @@ k_sweep(MeshView<FT> m, int step) {
     __shared__ FT sEx[2][S::NEX];
     __shared__ FT sEy[2][S::NEY];
 
+    // The block's 27 neighbour entries, read per lane from LDS rather than
+    // from the block table in HBM (they resolve the halo rows and columns
+    // of every plane).
+    __shared__ int sNbr[27];
+    if (tid < 27) sNbr[tid] = g.nbr[tid];
+    __syncthreads();
+
     // ---- primitive source of halo column (cx, cy) at plane z
     auto wsrc = [&](int cx, int cy, int z) -> const FT * {
         int p[3] = {x0 + cx, y0 + cy, z};
@@ -194,15 +201,32 @@ k_sweep(MeshView<FT> m, int step) {
         }
         int blk = b;
         if (out) {
-            blk = g.nbr[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)];
+            blk = sNbr[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)];
 #pragma unroll
             for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
         }
         return m.W + (size_t)blk * 8 * nc + cidx(m.n, p[0], p[1], p[2]);
     };
+    // normal face field of face (fx, fy, fz), component c: transverse
+    // indices may leave the block (same-level neighbour), the component's
+    // own index stays in 0 .. n
     auto bfn = [&](int c, int fx, int fy, int fz) -> FT {
-        const int p[3] = {x0 + fx, y0 + fy, fz};
-        return (FT)load_bf_nbr(m, b, step, c, p);
+        int p[3] = {x0 + fx, y0 + fy, fz};
+        int d[3] = {0, 0, 0};
+        bool out = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a == c) continue;
+            d[a] = p[a] < 0 ? -1 : (p[a] >= m.n[a] ? 1 : 0);
+            out |= d[a] != 0;
+        }
+        int blk = b;
+        if (out) {
+            blk = sNbr[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
+        }
+        return (FT)m.BF[(((size_t)blk * 2 + step) * 3 + c) * m.nface + fidx(m.n, p[0], p[1], p[2])];
     };
 
     // Asynchronous load of plane z of W into ring slot `slot` (no staging

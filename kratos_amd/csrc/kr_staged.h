@@ -176,30 +176,6 @@ __device__ __forceinline__ void load_cell(const MeshView<FT> &m, int b, const in
     }
 }
 
-// Normal face field at face p (component c) of a block whose neighbours are
-// all same-level blocks (no physical side, no level change): the fused
-// kernel's lean form of load_bf below.
-template <class FT>
-__device__ __forceinline__ double load_bf_nbr(const MeshView<FT> &m, int b, int slot, int c,
-                                              const int *pin) {
-    int p[3] = {pin[0], pin[1], pin[2]};
-    int d[3] = {0, 0, 0};
-    bool out = false;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (a == c) continue;
-        d[a] = p[a] < 0 ? -1 : (p[a] >= m.n[a] ? 1 : 0);
-        out |= d[a] != 0;
-    }
-    int blk = b;
-    if (out) {
-        blk = m.geo[b].nbr[nbr_dir(m, d)];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) p[a] -= d[a] * m.n[a];
-    }
-    return m.BF[(((size_t)blk * 2 + slot) * 3 + c) * m.nface + fidx(m.n, p[0], p[1], p[2])];
-}
-
 // Normal face field of block b at face p (component c); transverse indices
 // may sit one row outside the block (MHD transverse ghost lines), which the
 // reference fills by the tangential face-field exchange
