@@ -122,6 +122,7 @@ struct kr_ctx {
     bool mhd = false, mixed = false;
     bool fused = false;
     int variant = -1;               // kSweepVariants index of the fused kernel
+    int prod = -1;                  // refined mesh: producer tile (0: 16x8, 1: 8x8), -1 none
     std::vector<int> candidates;    // variants the autotuner times
     int tune_next = 0;              // next candidate to time
     bool tuned = false;
@@ -576,7 +577,16 @@ void commit(kr_ctx *c) {
         c->tuned = c->candidates.size() <= 1;
     }
     // cell-centred E for the staged EMF (c2p writes it only when allocated)
-    if (c->mhd && (!c->fused || c->smr)) KR_HIP(hipMalloc(&c->EC, nb * 3 * c->ncell * fs));
+    // producer form of the fused kernel for the blocks of a refined mesh
+    // that touch another level: 16x8 or 8x8 column tiles
+    c->prod = -1;
+    if (c->mhd && c->fused && c->smr && c->n[2] >= 4) {
+        if (c->n[0] % 16 == 0 && c->n[1] % 8 == 0) c->prod = 0;
+        else if (c->n[0] % 8 == 0 && c->n[1] % 8 == 0) c->prod = 1;
+    }
+    // cell-centred E for the staged EMF kernel, when it runs
+    if (c->mhd && (!c->fused || (c->smr && c->prod < 0)))
+        KR_HIP(hipMalloc(&c->EC, nb * 3 * c->ncell * fs));
     c->stats.n_blocks = (int)nb;
     c->stats.n_cells = (long long)(nb * c->ncell);
     // 5 state passes per Heun cycle of (5 conserved + 3 face-B) FP64 values
@@ -605,11 +615,16 @@ struct Launcher {
         evs.push_back({kind, {a, b}});
     }
 
-    template <class FT, int TX, int TY, int NT, int W>
+    template <class FT, int TX, int TY, int NT, int W, bool PROD = false>
     void sweep_t(const MeshView<FT> &m, int step, unsigned nb) {
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
-        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W>), dim3(nt, nb), dim3(NT), 0, c->stream,
+        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W, PROD>), dim3(nt, nb), dim3(NT), 0, c->stream,
                            m, step);
+    }
+    // fluxes and edge EMFs of the listed blocks into F / EE (no update)
+    template <class FT> void sweep_prod(const MeshView<FT> &m, int step, unsigned nb) {
+        if (c->prod == 0) sweep_t<FT, 16, 8, 256, 1, true>(m, step, nb);
+        else sweep_t<FT, 8, 8, 192, 1, true>(m, step, nb);
     }
 
     // grid rows: nb blocks (m.blist maps a row to its block when set)
@@ -677,6 +692,9 @@ struct Launcher {
             ns = (unsigned)c->nxblk;
         }
         if (ns == 0) return;
+        if (MHD && c->smr && c->prod >= 0) {
+            run(KR_K_FLUX_X, [&] { sweep_prod<FT>(ms, step, ns); });
+        } else {
         if (c->ndim > 2)
             run(KR_K_FLUX_Z, [&] {
                 hipLaunchKernelGGL((k_flux<FT, MHD, 2>), dim3(ceil_div(c->nfa[2], kThreads), ns),
@@ -696,6 +714,7 @@ struct Launcher {
                 hipLaunchKernelGGL(k_emf<FT>, dim3(ceil_div(c->nface, kThreads), ns), blk, 0,
                                    c->stream, ms);
             });
+        }
         if (c->smr) {                       // act_flx: EMF and flux restriction
             if (MHD && c->nxblk) {
                 run(KR_K_EMF, [&] {

@@ -145,7 +145,16 @@ __device__ __forceinline__ double edge_emf<double>(double ec11, double ec10, dou
 
 // WAVES > 1 asks the compiler for at least that many waves per SIMD (fewer
 // VGPRs, possibly spills); the tile autotuner times both forms.
-template <class FT, int TX, int TY, int NT, int WAVES>
+//
+// PROD (blocks of a refined mesh that touch another level): the kernel
+// only produces what the staged update needs -- the hydro fluxes of the
+// block's own faces into F and the edge EMFs of its own edges into EE, in
+// the staged kernels' layout and with the same values -- and stops before
+// the update, so that the cross-level corrections (restricted fluxes and
+// EMFs) can act on them before k_update.  Where a transverse ghost line
+// runs next to a face shared with another level, the normal field is the
+// cell value of W there (mhd/integrator/data.h:108-140), as in load_bf.
+template <class FT, int TX, int TY, int NT, int WAVES, bool PROD = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_sweep(MeshView<FT> m, int step) {
     using S = TileShape<TX, TY>;
@@ -209,8 +218,8 @@ k_sweep(MeshView<FT> m, int step) {
     };
     // normal face field of face (fx, fy, fz), component c: transverse
     // indices may leave the block (same-level neighbour), the component's
-    // own index stays in 0 .. n
-    auto bfn = [&](int c, int fx, int fy, int fz) -> FT {
+    // own index stays in 0 .. n.  `slot` is the ring slot of plane fz.
+    auto bfn = [&](int c, int fx, int fy, int fz, int slot) -> FT {
         int p[3] = {x0 + fx, y0 + fy, fz};
         int d[3] = {0, 0, 0};
         bool out = false;
@@ -221,6 +230,15 @@ k_sweep(MeshView<FT> m, int step) {
             out |= d[a] != 0;
         }
         int blk = b;
+        if constexpr (PROD) {
+            if (out && g.xface) {
+                int bit = 0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (d[a]) bit = 1 << (2 * a + (d[a] > 0));
+                if (g.xface & bit) return sW[slot][BCC + c][S::col(fx, fy)];
+            }
+        }
         if (out) {
             blk = sNbr[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)];
 #pragma unroll
@@ -320,7 +338,7 @@ k_sweep(MeshView<FT> m, int step) {
                         wl[v] = qL[v];
                         wr[v] = qR[v];
                     }
-                    const FT bn = bfn(2, zx, zy, p);
+                    const FT bn = bfn(2, zx, zy, p, s0);
                     wl[BCC + 2] = bn;
                     wr[BCC + 2] = bn;
                     rot<2>(wl + MOM);
@@ -337,7 +355,13 @@ k_sweep(MeshView<FT> m, int step) {
                     sFz[0][gz] = flx[RHO];
                     sFz[1][gz] = flx[BCC + 2];
                     sFz[2][gz] = -flx[BCC + 1];
-                    if (zint) {
+                    if (PROD && zint) {
+                        // the block's z face (x, y, p): hydro flux for the staged update
+                        const int q3[3] = {x0 + cx_, y0 + cy_, p};
+                        FT *F = m.F[2] + (size_t)b * 8 * m.nfa[2] + xidx(m.n, 2, q3);
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) F[(size_t)v * m.nfa[2]] = flx[v];
+                    } else if (zint) {
                         // z part of the cell finished now (plane p-1 = P)
                         if (P >= 0) {
                             const FT fa = (FT)(g.sfc[2] * dt_dv);
@@ -405,7 +429,7 @@ k_sweep(MeshView<FT> m, int step) {
                         wr[k] = q00 + (fr ? (FT)0 : -tr);
                     }
                 }
-                const FT bn = bfn(a, fx, fy, p);
+                const FT bn = bfn(a, fx, fy, p, s0);
                 wl[BCC] = bn;
                 wr[BCC] = bn;
                 if constexpr (std::is_same_v<FT, float>)
@@ -416,6 +440,19 @@ k_sweep(MeshView<FT> m, int step) {
                 sFh[0][it] = flx[ENE];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) sFh[1 + (a + k) % 3][it] = flx[MOM + k];
+                if constexpr (PROD) {
+                    // a face of the block (not a transverse ghost line, not a
+                    // ghost plane): hydro flux for the staged update
+                    if (!ghost_plane && (xf ? (fy >= 0 && fy < TY) : (fx >= 0 && fx < TX))) {
+                        const int q3[3] = {x0 + fx, y0 + fy, p};
+                        FT *F = m.F[a] + (size_t)b * 8 * m.nfa[a] + xidx(m.n, a, q3);
+                        F[(size_t)RHO * m.nfa[a]] = flx[RHO];
+                        F[(size_t)ENE * m.nfa[a]] = flx[ENE];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+                            F[(size_t)(MOM + (a + k) % 3) * m.nfa[a]] = flx[MOM + k];
+                    }
+                }
                 sFe[fe][0][it] = flx[RHO];
                 if (xf) {                        // x faces: E_y, E_z
                     sFe[fe][1][it] = flx[BCC + 2];
@@ -455,6 +492,8 @@ k_sweep(MeshView<FT> m, int step) {
                         sFez[S::fyi(x - 1, y)], sFez[S::fyi(x, y)],
                         sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
                     sEz[fe][S::ezi(x, y)] = e;
+                    if constexpr (PROD)
+                        m.EE[((size_t)b * 3 + 2) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 } else if (it < S::NEZ + S::NEX) {
                     const int k = it - S::NEZ;
                     const int x = k % TX, y = k / TX;
@@ -466,6 +505,8 @@ k_sweep(MeshView<FT> m, int step) {
                         sFz[1][S::zg(x, y - 1)], sFz[1][S::zg(x, y)],
                         sFe[fe][0][S::fyi(x, y)], sFe[fm][0][S::fyi(x, y)]);
                     sEx[fe][S::exi(x, y)] = e;
+                    if constexpr (PROD)
+                        m.EE[((size_t)b * 3 + 0) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 } else {
                     const int k = it - S::NEZ - S::NEX;
                     const int x = k % (TX + 1), y = k / (TX + 1);
@@ -477,6 +518,8 @@ k_sweep(MeshView<FT> m, int step) {
                         sFe[fm][1][S::fxi(x, y)], sFe[fe][1][S::fxi(x, y)],
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x - 1, y)]);
                     sEy[fe][S::eyi(x, y)] = e;
+                    if constexpr (PROD)
+                        m.EE[((size_t)b * 3 + 1) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 }
             }
         }
@@ -487,7 +530,7 @@ k_sweep(MeshView<FT> m, int step) {
         if (L + 1 <= nz + 1) issue_w(L + 1, ring3(L + 1));
 
         // ------------------------------------------------ 4. update
-        if (L >= 1) {
+        if (!PROD && L >= 1) {
             const int q = L - 1;                     // z face updated now
             const int fq = ring2(q), fpp = ring2(P);
             auto set_face = [&](int c, int fx, int fy, int fz, double db) {

@@ -156,10 +156,13 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrVie
 #pragma unroll
     for (int x = 0; x < 3; ++x) l[x] = 2 * G[x] - gf.reg[x] * m.n[x];
     l[a] = sd ? 0 : m.n[a];
+    // the hydro slots only: nothing reads the EMF slots of F after the EMF
+    // stage (and the fused kernel's producer form does not write them)
+    constexpr int NH = 5;
     const FT ds = (FT)gf.sfc[a];
-    FT fb[NV], norm = 0;
+    FT fb[NH], norm = 0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) fb[v] = 0;
+    for (int v = 0; v < NH; ++v) fb[v] = 0;
     const FT *Ff = m.F[a] + (size_t)f * NV * m.nfa[a];
 #pragma unroll
     for (int d1 = 0; d1 < 2; ++d1)
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrVie
             p[a2] += d2;
             const size_t x = xidx(m.n, a, p);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) fb[v] += Ff[(size_t)v * m.nfa[a] + x] * ds;
+            for (int v = 0; v < NH; ++v) fb[v] += Ff[(size_t)v * m.nfa[a] + x] * ds;
             norm += ds;
         }
     int q[3];
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrVie
     q[a2] = j2;
     FT *Fc = m.F[a] + (size_t)b * NV * m.nfa[a] + xidx(m.n, a, q);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) Fc[(size_t)v * m.nfa[a]] = fb[v] / norm;
+    for (int v = 0; v < NH; ++v) Fc[(size_t)v * m.nfa[a]] = fb[v] / norm;
 }
 
 // intg_ker: add the corrected flux of every refined face to the adjacent
