@@ -90,6 +90,14 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / s) if s > 0 else float(np.abs(a - b).sum())
 
 
+def divb_rel(divb, bfs, dx):
+    """max |div B| made dimensionless: times the cell width, over max |B|
+    of the face field (round-off of the face sums sits near 1e-16 times the
+    number of cycles' worth of accumulated updates)."""
+    bmax = max(float(np.abs(x).max()) for x in bfs)
+    return divb * dx / bmax if bmax > 0 else divb
+
+
 def sync_all(ctx):
     ctx.lib.kr_sync(ctx.h)
 
@@ -141,7 +149,9 @@ def accuracy_leg(n, nb, beta, t_lim, path, sensitivity=False):
     res = {"beta": beta, "t": tm, "cycles_mixed": cm, "cycles_fp64": cd,
            "l1_rho": rel_l1(um[0], ud[0]), "l1_E": rel_l1(um[1], ud[1]),
            "l1_B": rel_l1(cat(bm), cat(bd)),
-           "divb_max_mixed": divb_m, "divb_max_fp64": divb_d}
+           "divb_max_mixed": divb_m, "divb_max_fp64": divb_d,
+           "divb_rel_mixed": divb_rel(divb_m, bm, 1.0 / n),
+           "divb_rel_fp64": divb_rel(divb_d, bd, 1.0 / n)}
     if sensitivity:
         (up_, _), _, _, _ = out["fp64_perturbed"]
         res["l1_rho_fp64_perturbed_1e-12"] = rel_l1(up_[0], ud[0])
@@ -353,6 +363,7 @@ def main():
                           args.path, tile if form == "mixed" or tile[2] in (0, 1) else (0, 0, 0))
         r["rate"] = r["n_cells"] * r["cycles"] / r["wall"]
         r["divb"] = ctx.divb_max()
+        r["divb_rel"] = divb_rel(r["divb"], ctx.download()[1], 1.0 / n)
         r["t"] = ctx.time()[0]
         if fused is None:
             # the fused stage kernel reports under "update" with no flux launches
@@ -435,7 +446,7 @@ def main():
                    "form": head, "cells": r["n_cells"], "kernel_path": "fused" if fused else "staged"},
         "forms": {f: {"cell_updates_per_s": v["rate"],
                       "ms_per_cycle": v["wall"] / max(v["cycles"], 1) * 1e3,
-                      "divb_max": v["divb"],
+                      "divb_max": v["divb"], "divb_rel": v["divb_rel"],
                       "sweep_tile": v["sweep_tile"], "autotune_ms": v["tune_ms"],
                       "kernel_ms": v["prof"]["kernel_ms"] if v["prof"] else None}
                   for f, v in res.items()},
