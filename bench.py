@@ -33,7 +33,10 @@ roofline: the dominant kernel (largest share of the cycle) with its
 algorithmic bytes per launch divided by its mean launch time (HIP events on
 the library's stream).  traffic: HBM bytes per launch of that kernel from
 the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
-for the same kernel and workload, when present.  cpu_baseline: the CPU
+for the same kernel and workload, when present; valu_frac: the time that
+kernel's VALU instruction count (SQ_INSTS_VALU of the same summary) takes to
+issue at one wave64 instruction per 4 cycles per SIMD, over the launch time
+measured here.  cpu_baseline: the CPU
 oracle (FP64 C++ restatement of the reference algorithm, "port") on a
 bounded sample of the same recipe.
 """
@@ -289,17 +292,23 @@ def copy_probe():
                       "bytes_written_per_launch": 4 * n, "launches": 5}))
 
 
-def committed_traffic(kernel, workload):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
+def committed_profile(kernel, workload):
+    """The committed rocprofv3 PMC summary row of a kernel and workload:
+    HBM bytes and VALU wave-instructions per launch."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
         rows = json.load(f)
     for r in rows:
         if r.get("kernel") == kernel and r.get("workload") == workload:
-            return r.get("bytes_per_launch")
-    return None
+            return r
+    return {}
+
+
+# VALU issue rate of one MI355X: 256 CUs x 4 SIMDs, one wave64 instruction
+# per 4 cycles per SIMD (16 lanes; packed FP32 is not used), 2.4 GHz
+VALU_WAVE_INSTR_PER_S = 256 * 4 / 4 * 2.4e9
 
 
 def main():
@@ -398,12 +407,18 @@ def main():
         tl = r["sweep_tile"]
         kname = (f"k_sweep<{'float' if head == 'mixed' else 'double'},{tl[0]},{tl[1]},{tl[2]},{tl[3]}>"
                  if fused and dom == "update" else f"k_{dom}")
-        traffic = committed_traffic(kname, workload)
+        cp = committed_profile(kname, workload)
+        traffic = cp.get("bytes_per_launch")
+        valu = cp.get("valu_wave_instr_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": kname, "mean_launch_ms": round(kms[dom], 4),
                 "alg_bytes_per_launch": bytes_launch,
-                "cycle_frac": round(STATE_DOUBLES * 5 * 8 * r["rate"] / 1e9 / HBM_PEAK_GBS, 4)}
+                "cycle_frac": round(STATE_DOUBLES * 5 * 8 * r["rate"] / 1e9 / HBM_PEAK_GBS, 4),
+                # the VALU issue time of the launch's instruction count (from the
+                # committed counters) over the launch time measured here
+                "valu_frac": (round(valu / VALU_WAVE_INSTR_PER_S / (kms[dom] * 1e-3), 4)
+                              if valu else None)}
 
     acc = []
     if not args.no_accuracy:
