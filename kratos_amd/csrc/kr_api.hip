@@ -145,6 +145,8 @@ struct kr_ctx {
     SmrView sv{};
     int *d_table = nullptr;
     int4 *d_fill[2] = {nullptr, nullptr};   // restriction, prolongation tasks
+    int4 *d_pfill = nullptr;        // physical-side halo tasks {halo, block, direction, 0}
+    int npfill = 0;
     int nfill[2] = {0, 0};
     int2 *d_ftask = nullptr;        // faces whose neighbour is finer
     int nftask = 0;
@@ -196,11 +198,12 @@ void release(kr_ctx *c) {
     void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE, c->EC,
                     c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
                     c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk,
-                    c->d_pure};
+                    c->d_pure, c->d_pfill};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->d_table = nullptr;
     c->d_fill[0] = c->d_fill[1] = nullptr;
+    c->d_pfill = nullptr;
     c->d_ftask = nullptr;
     c->d_xblk = c->d_rblk = c->d_pure = nullptr;
     c->U = c->BF = nullptr;
@@ -514,7 +517,49 @@ void commit(kr_ctx *c) {
                 throw std::invalid_argument("refined MHD meshes are periodic");
         }
     }
-    // pools: W also holds the halo blocks of a refined mesh
+    // fused stage kernel: 3-D MHD blocks divisible by a supported tile
+    // (kr_config.tile forces a shape); the variant is picked by timing at
+    // the first step.  On a refined mesh the fused kernel takes the blocks
+    // with no cross-level direction (the others run its producer form and
+    // the staged update).
+    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
+        const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
+        c->candidates.clear();
+        for (int v = 0; v < kNumSweepVariants; ++v) {
+            const SweepVariant &sv = kSweepVariants[v];
+            if (tx && (sv.tx != tx || sv.ty != ty)) continue;
+            if (tw && sv.waves != tw) continue;
+            if (sv.waves > 1 && !c->mixed) continue;
+            if (c->n[0] % sv.tx == 0 && c->n[1] % sv.ty == 0 && c->n[2] >= 4)
+                c->candidates.push_back(v);
+        }
+        if (c->candidates.empty() && c->cfg.kernel_path == KR_PATH_FUSED)
+            throw std::invalid_argument("no fused tile fits the block shape");
+        c->fused = !c->candidates.empty();
+        if (c->fused) c->variant = c->candidates[0];
+        c->tune_next = 0;
+        c->tuned = c->candidates.size() <= 1;
+    }
+    // The fused kernel reads W through wnbr.  With physical sides, every
+    // direction of a block that crosses one gets a halo block in W: after
+    // c2p, k_phys_fill writes there the ghost primitives the boundary
+    // functors give (load_cell), at the positions a neighbour would hold.
+    for (auto &G : g) std::memcpy(G.wnbr, G.nbr, sizeof(G.nbr));
+    c->npfill = 0;
+    std::vector<int4> pfill;
+    bool phys_sides = false;
+    for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
+    if (c->fused && phys_sides && !c->smr) {
+        for (size_t b = 0; b < nb; ++b)
+            for (int k = 0; k < 27; ++k) {
+                if (k == 13 || g[b].nbr[k] >= 0) continue;
+                const int hb = (int)nb + c->nhalo++;
+                g[b].wnbr[k] = hb;
+                pfill.push_back(make_int4(hb, (int)b, k, 0));
+            }
+        c->npfill = (int)pfill.size();
+    }
+    // pools: W also holds the halo blocks (refined mesh, physical sides)
     const size_t fs = c->mixed ? sizeof(float) : sizeof(double);
     const size_t nw = nb + (size_t)c->nhalo;
     c->U = dalloc<double>(nb * 2 * 5 * c->ncell);
@@ -527,6 +572,10 @@ void commit(kr_ctx *c) {
     KR_HIP(hipMemset(c->W, 0, nw * c->nv * c->ncell * fs));
     c->geo = dalloc<BlockGeo>(nb);
     KR_HIP(hipMemcpy(c->geo, g.data(), nb * sizeof(BlockGeo), hipMemcpyHostToDevice));
+    if (c->npfill) {
+        c->d_pfill = dalloc<int4>(pfill.size());
+        KR_HIP(hipMemcpy(c->d_pfill, pfill.data(), pfill.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
 
     TimeState ts{};
     ts.t = c->cfg.t_0;
@@ -547,35 +596,6 @@ void commit(kr_ctx *c) {
     c->n_red_part = c->n_dt_part;
     c->red_part = dalloc<double>(c->n_red_part);
     c->red_out = dalloc<double>(1);
-    // fused stage kernel: 3-D MHD blocks divisible by a supported tile
-    // (kr_config.tile forces a shape); the variant is picked by timing at
-    // the first step
-    // (on a refined mesh the fused kernel takes the blocks with no
-    // cross-level direction; the others run the staged kernels)
-    // MHD physical sides run the staged kernels (the fused plane loads read
-    // neighbours only)
-    bool phys_sides = false;
-    for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
-    if (c->mhd && phys_sides && c->cfg.kernel_path == KR_PATH_FUSED)
-        throw std::invalid_argument("MHD physical sides run the staged kernels");
-    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED && !phys_sides) {
-        const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
-        c->candidates.clear();
-        for (int v = 0; v < kNumSweepVariants; ++v) {
-            const SweepVariant &sv = kSweepVariants[v];
-            if (tx && (sv.tx != tx || sv.ty != ty)) continue;
-            if (tw && sv.waves != tw) continue;
-            if (sv.waves > 1 && !c->mixed) continue;
-            if (c->n[0] % sv.tx == 0 && c->n[1] % sv.ty == 0 && c->n[2] >= 4)
-                c->candidates.push_back(v);
-        }
-        if (c->candidates.empty() && c->cfg.kernel_path == KR_PATH_FUSED)
-            throw std::invalid_argument("no fused tile fits the block shape");
-        c->fused = !c->candidates.empty();
-        if (c->fused) c->variant = c->candidates[0];
-        c->tune_next = 0;
-        c->tuned = c->candidates.size() <= 1;
-    }
     // cell-centred E for the staged EMF (c2p writes it only when allocated)
     // producer form of the fused kernel for the blocks of a refined mesh
     // that touch another level: 16x8 or 8x8 column tiles
@@ -657,6 +677,13 @@ struct Launcher {
             });
         if constexpr (MHD) {
             if (c->fused && !c->smr) {
+                if (c->npfill)              // physical-side halos (accounted as c2p)
+                    run(KR_K_C2P, [&] {
+                        const size_t mx = 4 * (size_t)std::max(c->n[0] * c->n[1],
+                                                               std::max(c->n[1] * c->n[2], c->n[0] * c->n[2]));
+                        hipLaunchKernelGGL((k_phys_fill<FT>), dim3(ceil_div(mx, 256), c->npfill),
+                                           dim3(256), 0, c->stream, m, (const int4 *)c->d_pfill);
+                    });
                 run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant, nb); }, true);
                 return;
             }

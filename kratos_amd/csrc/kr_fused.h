@@ -195,7 +195,7 @@ k_sweep(MeshView<FT> m, int step) {
     // from the block table in HBM (they resolve the halo rows and columns
     // of every plane).
     __shared__ int sNbr[27];
-    if (tid < 27) sNbr[tid] = g.nbr[tid];
+    if (tid < 27) sNbr[tid] = g.wnbr[tid];
     __syncthreads();
 
     // ---- primitive source of halo column (cx, cy) at plane z
@@ -230,6 +230,33 @@ k_sweep(MeshView<FT> m, int step) {
             out |= d[a] != 0;
         }
         int blk = b;
+        if (out && g.phy) {
+            // physical side (mhd/boundary/phys.h:178-229, phys_functors.h
+            // :10-137), as load_bf: the tangential field of the ghost cell
+            // next to the side -- free / outflow the boundary cell's, reflect
+            // minus the one at -i (two cells in), fixd the fixed field; along
+            // the component's own axis the row is clamped into the block where
+            // that axis is physical at the top
+            int a = 0;
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                if (d[x]) a = x;
+            if ((g.phy >> (2 * a + (d[a] > 0))) & 1) {
+                const int side = d[a] > 0;
+                const int kind = m.bnd[2 * a + side];
+                if (kind == 4) return (FT)m.fix_b[c];
+                double sgn = 1;
+                if (kind == 3) {
+                    p[a] = side ? m.n[a] - 2 : 1;
+                    sgn = -1;
+                } else {
+                    p[a] = side ? m.n[a] - 1 : 0;
+                }
+                if (((g.phy >> (2 * c + 1)) & 1) && p[c] >= m.n[c]) p[c] = m.n[c] - 1;
+                return (FT)(sgn * m.BF[(((size_t)b * 2 + step) * 3 + c) * m.nface +
+                                        fidx(m.n, p[0], p[1], p[2])]);
+            }
+        }
         if constexpr (PROD) {
             if (out && g.xface) {
                 int bit = 0;
@@ -480,17 +507,49 @@ k_sweep(MeshView<FT> m, int step) {
                 else
                     return b1 * v2 - b2 * v1;
             };
+            // boundary-plane EMF of a high out / fre side
+            // (mhd/integrator/integrator.h:328-381), as k_emf: edge c at
+            // tile position (x, y) and z index p takes the interior cell's
+            // value, v = w[mom] / rho as written there, the last such side
+            // (x, y, z order) winning
+            auto bplane = [&](int c, int x, int y, FT e) -> FT {
+                const int gi[3] = {x0 + x, y0 + y, p};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (a == c || gi[a] != m.n[a] || m.n[a] < 2 || !((g.phy >> (2 * a + 1)) & 1)) continue;
+                    const int k = m.bnd[2 * a + 1];
+                    if (k != 1 && k != 2) continue;
+                    const int cx = x - (a == 0), cy = y - (a == 1);
+                    const int sl = a == 2 ? sm : s0;
+                    const int cc = S::col(cx, cy);
+                    FT w[8];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) w[v] = sW[sl][v][cc];
+                    const FT irho = (FT)1 / pmax(w[RHO], (FT)1e-30);
+                    FT v_c[3], b_c[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        v_c[q] = w[MOM + q] * irho;
+                        b_c[q] = w[BCC + q];
+                    }
+                    const int a1 = mod3(a + 1), a2 = mod3(a + 2);
+                    if (c == a1) e = v_c[a] * b_c[a2] - v_c[a2] * b_c[a];
+                    else e = v_c[a1] * b_c[a] - v_c[a] * b_c[a1];
+                }
+                return e;
+            };
             for (int it = tid; it < S::NEZ + S::NEX + S::NEY; it += NT) {
                 if (it < S::NEZ) {
                     if (p >= nz) continue;
                     const int x = it % (TX + 1), y = it / (TX + 1);
                     // ee_z on plane p: a1 = x, a2 = y
-                    const FT e = edge_emf<FT>(
+                    FT e = edge_emf<FT>(
                         ec(s0, x, y, 2), ec(s0, x - 1, y, 2), ec(s0, x - 1, y - 1, 2), ec(s0, x, y - 1, 2),
                         sFez[S::fxi(x, y - 1)], sFez[S::fxi(x, y)],
                         sFe[fe][0][S::fyi(x, y)], sFe[fe][0][S::fyi(x - 1, y)],
                         sFez[S::fyi(x - 1, y)], sFez[S::fyi(x, y)],
                         sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
+                    if (g.phy) e = bplane(2, x, y, e);
                     sEz[fe][S::ezi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 2) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
@@ -498,12 +557,13 @@ k_sweep(MeshView<FT> m, int step) {
                     const int k = it - S::NEZ;
                     const int x = k % TX, y = k / TX;
                     // ee_x on z face p: a1 = y, a2 = z
-                    const FT e = edge_emf<FT>(
+                    FT e = edge_emf<FT>(
                         ec(s0, x, y, 0), ec(s0, x, y - 1, 0), ec(sm, x, y - 1, 0), ec(sm, x, y, 0),
                         sFe[fm][1][S::fyi(x, y)], sFe[fe][1][S::fyi(x, y)],
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x, y - 1)],
                         sFz[1][S::zg(x, y - 1)], sFz[1][S::zg(x, y)],
                         sFe[fe][0][S::fyi(x, y)], sFe[fm][0][S::fyi(x, y)]);
+                    if (g.phy) e = bplane(0, x, y, e);
                     sEx[fe][S::exi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 0) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
@@ -511,12 +571,13 @@ k_sweep(MeshView<FT> m, int step) {
                     const int k = it - S::NEZ - S::NEX;
                     const int x = k % (TX + 1), y = k / (TX + 1);
                     // ee_y on z face p: a1 = z, a2 = x
-                    const FT e = edge_emf<FT>(
+                    FT e = edge_emf<FT>(
                         ec(s0, x, y, 1), ec(sm, x, y, 1), ec(sm, x - 1, y, 1), ec(s0, x - 1, y, 1),
                         sFz[2][S::zg(x - 1, y)], sFz[2][S::zg(x, y)],
                         sFe[fe][0][S::fxi(x, y)], sFe[fm][0][S::fxi(x, y)],
                         sFe[fm][1][S::fxi(x, y)], sFe[fe][1][S::fxi(x, y)],
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x - 1, y)]);
+                    if (g.phy) e = bplane(1, x, y, e);
                     sEy[fe][S::eyi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 1) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
@@ -628,6 +689,37 @@ k_sweep(MeshView<FT> m, int step) {
             }
         }
     }
+}
+
+// Physical-side halo of the fused kernel.  One task {halo block, owner,
+// direction, 0} per grid row: the halo block's cells at the positions a
+// same-level neighbour in that direction would hold get the ghost
+// primitives that load_cell gives there -- the boundary functors
+// (hydro / mhd boundary/phys.h, phys_functors.h) the staged kernels apply
+// on the fly.  Runs after c2p, before the fused kernel reads the halo.
+template <class FT>
+__global__ void __launch_bounds__(256) k_phys_fill(MeshView<FT> m, const int4 *task) {
+    const int4 tk = task[blockIdx.y];
+    const int hb = tk.x, b = tk.y, dir = tk.z;
+    if (m.ts[m.geo[b].grp].skip) return;
+    const int d[3] = {dir % 3 - 1, (dir / 3) % 3 - 1, dir / 9 - 1};
+    int ext[3], lo[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = d[a] == 0 ? m.n[a] : 2;
+        lo[a] = d[a] < 0 ? m.n[a] - 2 : 0;          // halo-block local origin
+    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ext[0] * ext[1] * ext[2]) return;
+    const int hl[3] = {lo[0] + t % ext[0], lo[1] + (t / ext[0]) % ext[1], lo[2] + t / (ext[0] * ext[1])};
+    int p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = hl[a] + d[a] * m.n[a];
+    FT w[8];
+    load_cell<FT, 8>(m, b, p, w);
+    FT *Wh = m.W + (size_t)hb * 8 * m.ncell + cidx(m.n, hl[0], hl[1], hl[2]);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) Wh[(size_t)v * m.ncell] = w[v];
 }
 
 }  // namespace kr

@@ -19,6 +19,9 @@ struct BlockGeo {
     int nbr[27];       // block index at ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical side.
                        // On a refined mesh a direction whose neighbour region lies at
                        // another level points at a halo block (see SmrView).
+    int wnbr[27];      // W source of each direction for the fused kernel: nbr, except that a
+                       // direction crossing a physical side points at a halo block filled
+                       // with the boundary functors' ghost primitives (k_phys_fill)
     int phy;           // bit (2*axis + side) set on a physical side
     int grp;           // time group: index of this block's TimeState
     // refined meshes (all zero on a same-level mesh)

@@ -269,13 +269,17 @@ def test_mhd_turbulence_128_fp64_100_cycles(gpu_lib):
 @pytest.mark.parametrize("bnd", [("out", "out", "ref", "ref", "fre", "fre"),
                                  ("ref", "fre", "per", "per", "out", "ref"),
                                  ("fre", "fre", "out", "out", "out", "out")])
-def test_mhd_physical_sides_fp64(gpu_lib, bnd):
-    """MHD with outflow / free / reflecting sides (staged kernels), 8 blocks,
-    bit-identical with the oracle after 10 cycles; div B at round-off."""
+@pytest.mark.parametrize("path", ["staged", "fused"])
+def test_mhd_physical_sides_fp64(gpu_lib, bnd, path):
+    """MHD with outflow / free / reflecting sides, 8 blocks, bit-identical
+    with the oracle after 10 cycles; div B at round-off.  Staged: the
+    functors applied where cells are read; fused: the ghost primitives in
+    halo blocks filled after c2p, the face field and the boundary-plane EMF
+    in the kernel."""
     spec = kratos_amd.MeshSpec((32, 32, 32), (16, 16, 16), bnd=bnd)
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     u, b, _ = problems.turbulence((32, 32, 32), seed=12, beta=2.0)
-    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b, 10, path=path)
     assert gt[0] == ot[0]
     np.testing.assert_array_equal(gu, ou)
     for x, y in zip(gb, ob):
@@ -298,8 +302,8 @@ def test_mhd_physical_sides_mixed(gpu_lib, bnd, bar):
     assert rel_l1(gu[0], ou[0]) <= bar
 
 
-@pytest.mark.parametrize("mhd", [False, True])
-def test_fixed_sides_fp64(gpu_lib, mhd):
+@pytest.mark.parametrize("mhd,path", [(False, "auto"), (True, "staged"), (True, "fused")])
+def test_fixed_sides_fp64(gpu_lib, mhd, path):
     """"fixd" (Dirichlet) sides holding an ambient state, hydro and MHD,
     bit-identical with the oracle after 10 cycles."""
     spec = kratos_amd.MeshSpec((32, 32, 32), (16, 16, 16),
@@ -307,7 +311,7 @@ def test_fixed_sides_fp64(gpu_lib, mhd):
     phys = kratos_amd.Physics(mhd=mhd, gamma=5.0 / 3.0, cfl=0.3,
                               fix_w=(1.1, 0.7, 0.3, -0.2, 0.1), fix_b=(0.4, -0.3, 0.2))
     u, b, _ = problems.turbulence((32, 32, 32), seed=14, beta=2.0)
-    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b if mhd else None, 10)
+    (gu, gb, gt), (ou, ob, ot) = run_both(spec, phys, u, b if mhd else None, 10, path=path)
     assert gt[0] == ot[0]
     np.testing.assert_array_equal(gu, ou)
     if mhd:
