@@ -4,8 +4,10 @@ every conserved value and face field bit-identical; mixed form: density L1
 <= 1e-5 relative.  Cases cover hydro and MHD, two and three levels, a
 physical (outflow / reflecting) hydro box, and a refined mesh whose blocks
 all sit on level 0 (the same-level path fed by the refined-mesh builder).
-MHD runs the fused kernel on blocks with no cross-level direction and the
-staged kernels on the others (and everything staged when asked)."""
+MHD runs the fused kernel on blocks with no cross-level direction and its
+producer form (fluxes and EMFs into HBM) plus the staged update on the
+others (and everything staged when asked); 8^3 blocks take the 8x8
+producer tile, 16^3 blocks the 16x8 one."""
 import numpy as np
 import pytest
 
@@ -66,6 +68,16 @@ def test_smr_two_levels_fp64(gpu_lib, mhd, path):
     _assert_equal(spec, g, o, mhd)
     if mhd:
         assert divb < 1e-11
+
+
+def test_smr_16_blocks_mhd_fp64(gpu_lib):
+    """16^3 blocks: the 16x8 producer tile on the blocks at level changes."""
+    spec = RefinedMeshSpec((64, 64, 64), (16, 16, 16), refine=TWO)
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    g, o, gt, ot, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=8, mhd=True), 4)
+    assert gt[0] == ot[0]
+    _assert_equal(spec, g, o, True)
+    assert divb < 1e-11
 
 
 def test_smr_three_levels_mhd_fp64(gpu_lib):
