@@ -20,6 +20,9 @@ case the parity tests run), and max |div B|.  At beta = 0.01 also the L1
 distance that a 1e-12 relative perturbation of the initial density reaches
 in the FP64 form alone: the problem's own sensitivity floor.
 
+hydro: the hydro module (PLM + HLLC + RK2) on the same 256^3 recipe without
+the face field, 512 blocks of 32^3, staged kernels, both forms.
+
 smr_blast: the MHD blast on a three-level statically refined mesh (128^3
 root cells in 16^3 blocks, ~2,000 blocks), both forms on the staged kernels
 with the cross-level corrections; cell-updates/s counts every block's cells.
@@ -200,6 +203,43 @@ def ensemble_leg(n_box, n, t_lim, path, nb=None):
             "divb_max_mixed": res["mixed"]["divb"]}
 
 
+def hydro_leg(n, nb, steps, warmup, path):
+    """The hydro module on the same recipe (the turbulence state without
+    the face field, gamma 5/3) on the staged kernels: cell-updates/s of
+    both forms, per-cycle kernel times, and the mixed form's density L1
+    against FP64 after warmup + timed cycles.  cycle_hbm_frac prices a
+    cycle at 200 B per cell (stage 1 reads U and writes U', stage 2 reads
+    U', U and writes U: five passes of 5 conserved doubles), the least
+    any two-stage scheme moves."""
+    import kratos_amd
+    spec = kratos_amd.MeshSpec((n, n, n), (nb, nb, nb))
+    phys = kratos_amd.Physics(mhd=False, gamma=5.0 / 3.0, cfl=0.3)
+    u, _, _ = make_problem(n)
+    out = {"workload": f"hydro turbulence {n}^3, {spec.n_blocks} blocks of {nb}^3"}
+    states = {}
+    for form in ("fp64", "mixed"):
+        ctx = kratos_amd.Context(spec, phys, form, path=path)
+        ctx.upload(u, None)
+        ctx.advance(warmup)
+        sync_all(ctx)
+        t0 = time.perf_counter()
+        done = ctx.advance(steps)
+        sync_all(ctx)
+        wall = time.perf_counter() - t0
+        rate = n ** 3 * done / wall
+        out[f"cell_updates_per_s_{form}"] = rate
+        out[f"ms_per_cycle_{form}"] = wall / max(done, 1) * 1e3
+        out[f"cycle_hbm_frac_{form}"] = round(200 * rate / 1e9 / HBM_PEAK_GBS, 4)
+        states[form] = ctx.download()[0]
+        prof = ctx.profile(2)
+        out[f"kernel_ms_per_cycle_{form}"] = {
+            k: round(v * prof["kernel_calls"][k] / 2, 4) for k, v in prof["kernel_ms"].items()}
+        ctx.close()
+    out["l1_rho_mixed_vs_fp64"] = rel_l1(states["mixed"][0], states["fp64"][0])
+    out["speedup_mixed_over_fp64"] = out["cell_updates_per_s_mixed"] / out["cell_updates_per_s_fp64"]
+    return out
+
+
 def smr_leg(n_root, nb, steps, warmup, cycles_acc):
     """MHD blast on a statically refined mesh: [-0.5, 0.5]^3 periodic,
     n_root^3 root cells in nb^3 blocks, level 1 on |x_a| <= 0.25 and level 2
@@ -331,6 +371,7 @@ def main():
     ap.add_argument("--ens-n", type=int, default=64)
     ap.add_argument("--t-ens", type=float, default=0.05)
     ap.add_argument("--no-smr", action="store_true")
+    ap.add_argument("--no-hydro", action="store_true")
     ap.add_argument("--smr-only", action="store_true",
                     help="run only the refined-mesh blast leg and print its JSON")
     ap.add_argument("--smr-root", type=int, default=128)
@@ -438,6 +479,10 @@ def main():
                                                       args.path, args.ens_n // 2)
             print(f"[bench] ensemble, 8 blocks per box: {ens['as_8_blocks_per_box']}",
                   file=sys.stderr, flush=True)
+    hyd = None
+    if not args.no_hydro:
+        hyd = hydro_leg(n, nb, args.steps, args.warmup, args.path)
+        print(f"[bench] hydro: {hyd}", file=sys.stderr, flush=True)
     smr = None
     if not args.no_smr:
         smr = smr_leg(args.smr_root, args.smr_block, max(args.steps // 2, 4), 2, 20)
@@ -470,6 +515,7 @@ def main():
         "accuracy_mixed_vs_fp64": acc,
         "ensemble": ens,
         "smr_blast": smr,
+        "hydro": hyd,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
