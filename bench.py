@@ -21,7 +21,7 @@ distance that a 1e-12 relative perturbation of the initial density reaches
 in the FP64 form alone: the problem's own sensitivity floor.
 
 hydro: the hydro module (PLM + HLLC + RK2) on the same 256^3 recipe without
-the face field, 512 blocks of 32^3, staged kernels, both forms.
+the face field, 512 blocks of 32^3, both forms (fused hydro kernel).
 
 smr_blast: the MHD blast on a three-level statically refined mesh (128^3
 root cells in 16^3 blocks, ~2,000 blocks), both forms on the staged kernels
@@ -205,9 +205,9 @@ def ensemble_leg(n_box, n, t_lim, path, nb=None):
 
 def hydro_leg(n, nb, steps, warmup, path):
     """The hydro module on the same recipe (the turbulence state without
-    the face field, gamma 5/3) on the staged kernels: cell-updates/s of
-    both forms, per-cycle kernel times, and the mixed form's density L1
-    against FP64 after warmup + timed cycles.  cycle_hbm_frac prices a
+    the face field, gamma 5/3; the fused hydro kernel under path auto):
+    cell-updates/s of both forms, per-cycle kernel times, and the mixed
+    form's density L1 against FP64 after warmup + timed cycles.  cycle_hbm_frac prices a
     cycle at 200 B per cell (stage 1 reads U and writes U', stage 2 reads
     U', U and writes U: five passes of 5 conserved doubles), the least
     any two-stage scheme moves."""

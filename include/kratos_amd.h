@@ -50,8 +50,9 @@ typedef struct kr_config {
     double t_0, t_lim, dt_init, dt_expand;     /* [cycle] keys             */
     int    top_up;         /* [cycle] t_lim_top_up                            */
     int    n_cell_block[3];
-    int    kernel_path;    /* kr_path: KR_PATH_AUTO picks the fused MHD stage
-                              kernel when the block shape allows it         */
+    int    kernel_path;    /* kr_path: KR_PATH_AUTO picks the fused stage
+                              kernel (MHD; hydro on one-level meshes) when
+                              the block shape allows it                     */
     int    tile[3];        /* fused-kernel column tile (x, y) and minimum
                               waves per SIMD; 0 entries are left to the
                               autotuner, which runs the first cycles with

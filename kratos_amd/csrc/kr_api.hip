@@ -3,6 +3,7 @@
 // the hipGraph replay of whole cycles.
 #include "../../include/kratos_amd.h"
 #include "kr_fused.h"
+#include "kr_fused_hydro.h"
 #include "kr_smr.h"
 
 #include <hip/hip_runtime.h>
@@ -517,19 +518,19 @@ void commit(kr_ctx *c) {
                 throw std::invalid_argument("refined MHD meshes are periodic");
         }
     }
-    // fused stage kernel: 3-D MHD blocks divisible by a supported tile
+    // fused stage kernel: 3-D blocks divisible by a supported tile
     // (kr_config.tile forces a shape); the variant is picked by timing at
-    // the first step.  On a refined mesh the fused kernel takes the blocks
-    // with no cross-level direction (the others run its producer form and
-    // the staged update).
-    if (c->mhd && c->cfg.kernel_path != KR_PATH_STAGED) {
+    // the first step.  On a refined MHD mesh the fused kernel takes the
+    // blocks with no cross-level direction (the others run its producer form
+    // and the staged update); refined hydro meshes run the staged kernels.
+    if ((c->mhd || !c->smr) && c->cfg.kernel_path != KR_PATH_STAGED) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {
             const SweepVariant &sv = kSweepVariants[v];
             if (tx && (sv.tx != tx || sv.ty != ty)) continue;
             if (tw && sv.waves != tw) continue;
-            if (sv.waves > 1 && !c->mixed) continue;
+            if (sv.waves > 1 && (!c->mixed || !c->mhd)) continue;
             if (c->n[0] % sv.tx == 0 && c->n[1] % sv.ty == 0 && c->n[2] >= 4)
                 c->candidates.push_back(v);
         }
@@ -559,6 +560,13 @@ void commit(kr_ctx *c) {
             }
         c->npfill = (int)pfill.size();
     }
+    // a direction with neither a neighbour nor a halo block (an edge or
+    // corner of a hydro mesh given face neighbours only) reads the block
+    // itself: those halo columns are staged by the fused kernel but enter
+    // no hydro flux, and every staged address stays inside W
+    for (size_t b = 0; b < nb; ++b)
+        for (int k = 0; k < 27; ++k)
+            if (g[b].wnbr[k] < 0) g[b].wnbr[k] = (int)b;
     // pools: W also holds the halo blocks (refined mesh, physical sides)
     const size_t fs = c->mixed ? sizeof(float) : sizeof(double);
     const size_t nw = nb + (size_t)c->nhalo;
@@ -647,6 +655,22 @@ struct Launcher {
         else sweep_t<FT, 8, 8, 192, 1, true>(m, step, nb);
     }
 
+    template <class FT, int TX, int TY, int NT>
+    void sweep_hyd_t(const MeshView<FT> &m, int step, unsigned nb) {
+        const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
+        hipLaunchKernelGGL((k_sweep_hyd<FT, TX, TY, NT>), dim3(nt, nb), dim3(NT), 0, c->stream, m,
+                           step);
+    }
+    // hydro: the one-wave-per-SIMD-minimum variants of the tile table
+    template <class FT> void sweep_hyd(const MeshView<FT> &m, int step, int variant, unsigned nb) {
+        switch (variant) {
+        case 0: sweep_hyd_t<FT, 32, 8, 512>(m, step, nb); break;
+        case 2: sweep_hyd_t<FT, 16, 16, 512>(m, step, nb); break;
+        case 5: sweep_hyd_t<FT, 8, 8, 192>(m, step, nb); break;
+        default: sweep_hyd_t<FT, 16, 8, 256>(m, step, nb); break;
+        }
+    }
+
     // grid rows: nb blocks (m.blist maps a row to its block when set)
     template <class FT> void sweep(const MeshView<FT> &m, int step, int variant, unsigned nb) {
         constexpr bool f32 = std::is_same_v<FT, float>;
@@ -675,20 +699,21 @@ struct Launcher {
                                    (const double *)c->dt_part, (const int *)c->grp_first,
                                    c->ppb);
             });
-        if constexpr (MHD) {
-            if (c->fused && !c->smr) {
-                if (c->npfill)              // physical-side halos (accounted as c2p)
-                    run(KR_K_C2P, [&] {
-                        const size_t mx = 4 * (size_t)std::max(c->n[0] * c->n[1],
-                                                               std::max(c->n[1] * c->n[2], c->n[0] * c->n[2]));
-                        hipLaunchKernelGGL((k_phys_fill<FT>), dim3(ceil_div(mx, 256), c->npfill),
-                                           dim3(256), 0, c->stream, m, (const int4 *)c->d_pfill);
-                    });
-                run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant, nb); }, true);
-                return;
-            }
-        }
         constexpr int NV = MHD ? 8 : 5;
+        if (c->fused && !c->smr) {
+            if (c->npfill)                  // physical-side halos (accounted as c2p)
+                run(KR_K_C2P, [&] {
+                    const size_t mx = 4 * (size_t)std::max(c->n[0] * c->n[1],
+                                                           std::max(c->n[1] * c->n[2], c->n[0] * c->n[2]));
+                    hipLaunchKernelGGL((k_phys_fill<FT, NV>), dim3(ceil_div(mx, 256), c->npfill),
+                                       dim3(256), 0, c->stream, m, (const int4 *)c->d_pfill);
+                });
+            if constexpr (MHD)
+                run(KR_K_UPDATE, [&] { sweep<FT>(m, step, c->variant, nb); }, true);
+            else
+                run(KR_K_UPDATE, [&] { sweep_hyd<FT>(m, step, c->variant, nb); }, true);
+            return;
+        }
         if (c->smr) {                       // halo where levels meet (accounted as c2p)
             size_t mx = 0;                  // largest face region: 2 x n x n
             for (int a = 0; a < 3; ++a) {

@@ -697,7 +697,8 @@ k_sweep(MeshView<FT> m, int step) {
 // primitives that load_cell gives there -- the boundary functors
 // (hydro / mhd boundary/phys.h, phys_functors.h) the staged kernels apply
 // on the fly.  Runs after c2p, before the fused kernel reads the halo.
-template <class FT>
+// NV: 8 for MHD, 5 for hydro.
+template <class FT, int NV>
 __global__ void __launch_bounds__(256) k_phys_fill(MeshView<FT> m, const int4 *task) {
     const int4 tk = task[blockIdx.y];
     const int hb = tk.x, b = tk.y, dir = tk.z;
@@ -715,11 +716,11 @@ __global__ void __launch_bounds__(256) k_phys_fill(MeshView<FT> m, const int4 *t
     int p[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[a] = hl[a] + d[a] * m.n[a];
-    FT w[8];
-    load_cell<FT, 8>(m, b, p, w);
-    FT *Wh = m.W + (size_t)hb * 8 * m.ncell + cidx(m.n, hl[0], hl[1], hl[2]);
+    FT w[NV];
+    load_cell<FT, NV>(m, b, p, w);
+    FT *Wh = m.W + (size_t)hb * NV * m.ncell + cidx(m.n, hl[0], hl[1], hl[2]);
 #pragma unroll
-    for (int v = 0; v < 8; ++v) Wh[(size_t)v * m.ncell] = w[v];
+    for (int v = 0; v < NV; ++v) Wh[(size_t)v * m.ncell] = w[v];
 }
 
 }  // namespace kr
