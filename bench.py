@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--t-ens", type=float, default=0.05)
     ap.add_argument("--no-smr", action="store_true")
     ap.add_argument("--no-hydro", action="store_true")
+    ap.add_argument("--hydro-only", action="store_true",
+                    help="run only the hydro leg and print its JSON object")
     ap.add_argument("--smr-only", action="store_true",
                     help="run only the refined-mesh blast leg and print its JSON")
     ap.add_argument("--smr-root", type=int, default=128)
@@ -389,6 +391,11 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding is designed but not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.hydro_only:
+        hyd = hydro_leg(args.n, args.block, args.steps, args.warmup, args.path)
+        print(json.dumps({k: (float(v) if isinstance(v, np.floating) else v) for k, v in hyd.items()}))
+        return
 
     if args.smr_only:
         smr = smr_leg(args.smr_root, args.smr_block, max(args.steps // 2, 4), args.warmup, 20)
