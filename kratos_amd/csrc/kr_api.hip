@@ -520,10 +520,13 @@ void commit(kr_ctx *c) {
     }
     // fused stage kernel: 3-D blocks divisible by a supported tile
     // (kr_config.tile forces a shape); the variant is picked by timing at
-    // the first step.  On a refined MHD mesh the fused kernel takes the
-    // blocks with no cross-level direction (the others run its producer form
-    // and the staged update); refined hydro meshes run the staged kernels.
-    if ((c->mhd || !c->smr) && c->cfg.kernel_path != KR_PATH_STAGED) {
+    // the first step.  On a refined mesh the fused kernel takes the blocks
+    // with no cross-level direction; the others run, for MHD, its producer
+    // form and the staged update, for hydro the staged kernels.  Refined
+    // hydro meshes with physical sides run the staged kernels throughout.
+    bool phys_sides = false;
+    for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
+    if ((c->mhd || !c->smr || !phys_sides) && c->cfg.kernel_path != KR_PATH_STAGED) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {
@@ -548,8 +551,6 @@ void commit(kr_ctx *c) {
     for (auto &G : g) std::memcpy(G.wnbr, G.nbr, sizeof(G.nbr));
     c->npfill = 0;
     std::vector<int4> pfill;
-    bool phys_sides = false;
-    for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
     if (c->fused && phys_sides && !c->smr) {
         for (size_t b = 0; b < nb; ++b)
             for (int k = 0; k < 27; ++k) {
@@ -734,11 +735,15 @@ struct Launcher {
         // kernel the blocks that touch another level (the rest run fused)
         MeshView<FT> ms = m;
         unsigned ns = nb;
-        if (MHD && c->smr && c->fused) {
+        if (c->smr && c->fused) {
             if (c->npure) {
                 MeshView<FT> mp = m;
                 mp.blist = c->d_pure;
-                run(KR_K_UPDATE, [&] { sweep<FT>(mp, step, c->variant, (unsigned)c->npure); }, true);
+                if constexpr (MHD)
+                    run(KR_K_UPDATE, [&] { sweep<FT>(mp, step, c->variant, (unsigned)c->npure); }, true);
+                else
+                    run(KR_K_UPDATE, [&] { sweep_hyd<FT>(mp, step, c->variant, (unsigned)c->npure); },
+                        true);
             }
             ms.blist = c->d_xblk;
             ns = (unsigned)c->nxblk;
