@@ -647,8 +647,12 @@ struct Launcher {
     template <class FT, int TX, int TY, int NT, int W, bool PROD = false>
     void sweep_t(const MeshView<FT> &m, int step, unsigned nb) {
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
-        hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W, PROD>), dim3(nt, nb), dim3(NT), 0, c->stream,
-                           m, step);
+        if (c->npfill)                      // physical sides (one-level meshes only)
+            hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W, PROD, true>), dim3(nt, nb), dim3(NT), 0,
+                               c->stream, m, step);
+        else
+            hipLaunchKernelGGL((k_sweep<FT, TX, TY, NT, W, PROD, false>), dim3(nt, nb), dim3(NT), 0,
+                               c->stream, m, step);
     }
     // fluxes and edge EMFs of the listed blocks into F / EE (no update)
     template <class FT> void sweep_prod(const MeshView<FT> &m, int step, unsigned nb) {

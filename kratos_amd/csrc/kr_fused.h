@@ -146,6 +146,12 @@ __device__ __forceinline__ double edge_emf<double>(double ec11, double ec10, dou
 // WAVES > 1 asks the compiler for at least that many waves per SIMD (fewer
 // VGPRs, possibly spills); the tile autotuner times both forms.
 //
+// PHY: the mesh has physical sides, and the blocks next to them apply the
+// boundary rules of the face field and of the boundary-plane EMF (the ghost
+// primitives come from halo blocks, as a neighbour's would).  Periodic
+// meshes take the build without those branches, which keeps their
+// registers and code out of the common path.
+//
 // PROD (blocks of a refined mesh that touch another level): the kernel
 // only produces what the staged update needs -- the hydro fluxes of the
 // block's own faces into F and the edge EMFs of its own edges into EE, in
@@ -154,7 +160,7 @@ __device__ __forceinline__ double edge_emf<double>(double ec11, double ec10, dou
 // EMFs) can act on them before k_update.  Where a transverse ghost line
 // runs next to a face shared with another level, the normal field is the
 // cell value of W there (mhd/integrator/data.h:108-140), as in load_bf.
-template <class FT, int TX, int TY, int NT, int WAVES, bool PROD = false>
+template <class FT, int TX, int TY, int NT, int WAVES, bool PROD = false, bool PHY = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_sweep(MeshView<FT> m, int step) {
     using S = TileShape<TX, TY>;
@@ -230,7 +236,7 @@ k_sweep(MeshView<FT> m, int step) {
             out |= d[a] != 0;
         }
         int blk = b;
-        if (out && g.phy) {
+        if (PHY && out && g.phy) {
             // physical side (mhd/boundary/phys.h:178-229, phys_functors.h
             // :10-137), as load_bf: the tangential field of the ghost cell
             // next to the side -- free / outflow the boundary cell's, reflect
@@ -549,7 +555,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFe[fe][0][S::fyi(x, y)], sFe[fe][0][S::fyi(x - 1, y)],
                         sFez[S::fyi(x - 1, y)], sFez[S::fyi(x, y)],
                         sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
-                    if (g.phy) e = bplane(2, x, y, e);
+                    if (PHY && g.phy) e = bplane(2, x, y, e);
                     sEz[fe][S::ezi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 2) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
@@ -563,7 +569,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x, y - 1)],
                         sFz[1][S::zg(x, y - 1)], sFz[1][S::zg(x, y)],
                         sFe[fe][0][S::fyi(x, y)], sFe[fm][0][S::fyi(x, y)]);
-                    if (g.phy) e = bplane(0, x, y, e);
+                    if (PHY && g.phy) e = bplane(0, x, y, e);
                     sEx[fe][S::exi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 0) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
@@ -577,7 +583,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFe[fe][0][S::fxi(x, y)], sFe[fm][0][S::fxi(x, y)],
                         sFe[fm][1][S::fxi(x, y)], sFe[fe][1][S::fxi(x, y)],
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x - 1, y)]);
-                    if (g.phy) e = bplane(1, x, y, e);
+                    if (PHY && g.phy) e = bplane(1, x, y, e);
                     sEy[fe][S::eyi(x, y)] = e;
                     if constexpr (PROD)
                         m.EE[((size_t)b * 3 + 1) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
