@@ -7,7 +7,7 @@ leg of ``bench.py`` may import this package, and only as the checker; the
 kratos_amd product never does.
 
 Parity status of each oracle function (how it is pinned) is listed in the
-header comment of each test module under tests/ and summarised in DESIGN.md.
+header comment of each test module under tests/.
 """
 import ctypes as C
 import os
