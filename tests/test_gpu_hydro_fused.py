@@ -121,6 +121,35 @@ def test_hydro_mixed_fused_equals_staged(gpu_lib):
     np.testing.assert_array_equal(res["fused"], res["staged"])
 
 
+@pytest.mark.parametrize("path", ["staged", "fused"])
+def test_hydro_ensemble_per_box_dt(gpu_lib, path):
+    """Independent periodic hydro boxes in one context, each its own time
+    group (own dt, own stop at t_lim; a finished box's workgroups return at
+    once): each equals an oracle run of that box alone, bit for bit."""
+    N = (16, 16, 16)
+    spec = kratos_amd.MeshSpec(N, N)
+    phys = kratos_amd.Physics(mhd=False, gamma=G, cfl=0.3, t_lim=0.03, top_up=True)
+    ics = [problems.turbulence(N, seed=s, mach=mc)[0] for s, mc in enumerate((0.2, 0.5, 1.0, 2.0))]
+    print(f"\n[case] hydro ensemble {len(ics)} x {N} path={path}", flush=True)
+    ctx = kratos_amd.Context(spec, phys, "fp64", path=path, members=len(ics))
+    for k, u in enumerate(ics):
+        ctx.upload(u, None, member=k)
+    ctx.advance(100000)
+    cycles = []
+    for k, u in enumerate(ics):
+        gu, _ = ctx.download(member=k)
+        t, dt, cyc = ctx.time(member=k)
+        cycles.append(cyc)
+        ref = oracle.OracleMesh(spec, phys)
+        ref.upload(u, None)
+        ref.advance(100000)
+        ou, _ = ref.download()
+        assert t == ref.time()[0] == 0.03
+        np.testing.assert_array_equal(gu, ou)
+    ctx.close()
+    assert len(set(cycles)) > 1, "boxes should need different cycle counts"
+
+
 def test_hydro_fused_needs_3d_tile(gpu_lib):
     """Blocks no tile divides, 2-D meshes and the waves-per-SIMD variants
     (an MHD-only form) cannot take the fused hydro path: the context
