@@ -39,7 +39,9 @@ the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/
 for the same kernel and workload, when present; valu_frac: the time that
 kernel's VALU instruction count (SQ_INSTS_VALU of the same summary) takes to
 issue at one wave64 instruction per 4 cycles per SIMD, over the launch time
-measured here.  cpu_baseline: the CPU
+measured here; stream_copy_gbs: a plain 1 GiB device copy timed on the
+same GPU, the rate a streaming kernel reaches there (frac_of_stream_copy
+puts the dominant kernel against it).  cpu_baseline: the CPU
 oracle (FP64 C++ restatement of the reference algorithm, "port") on a
 bounded sample of the same recipe.
 """
@@ -332,6 +334,27 @@ def copy_probe():
                       "bytes_written_per_launch": 4 * n, "launches": 5}))
 
 
+def stream_copy_gbs(n_bytes=1 << 30, launches=20):
+    """Measured HBM rate of a plain device copy on this GPU (torch.mul(a,
+    1.0, out=b) over n_bytes in and n_bytes out, timed with events on
+    torch's stream): what a streaming kernel reaches here, beside the spec
+    sheet's 8 TB/s."""
+    import torch
+    a = torch.ones(n_bytes // 4, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(launches):
+        torch.mul(a, 1.0, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / launches
+    del a, b
+    return 2 * n_bytes / (ms * 1e-3) / 1e9
+
+
 def committed_profile(kernel, workload):
     """The committed rocprofv3 PMC summary row of a kernel and workload:
     HBM bytes and VALU wave-instructions per launch."""
@@ -467,6 +490,9 @@ def main():
                 # committed counters) over the launch time measured here
                 "valu_frac": (round(valu / VALU_WAVE_INSTR_PER_S / (kms[dom] * 1e-3), 4)
                               if valu else None)}
+        copy = stream_copy_gbs()
+        roof["stream_copy_gbs"] = round(copy, 1)
+        roof["frac_of_stream_copy"] = round(achieved / copy, 4)
 
     acc = []
     if not args.no_accuracy:
