@@ -182,4 +182,82 @@ __device__ __forceinline__ void hlld_fast(float *flx, const float *wl, const flo
     flx[BCC + 2] = g[6];
 }
 
+// HLLC of the mixed form (hydro/riemann/hllc.h:76-193, the wave speeds,
+// contact speed and FLT-constant guards of hllc<FT>) with the hardware
+// reciprocal and square root in place of the IEEE division / square-root
+// sequences, 1/(gamma-1) and 1/rho formed once per side.  No multiply-add
+// contraction here: every call site (the fused kernel's z and in-plane
+// faces, the staged flux kernel) rounds the same way, so a face shared by
+// two of them gets one flux and the update stays conservative.
+__device__ __forceinline__ void hllc_fast(float *flx, const float *wl, const float *wr, float gam) {
+#pragma clang fp contract(off)
+    const float cv_kb = frcp(gam - 1.f);
+    const float irl = frcp(wl[RHO]), irr = frcp(wr[RHO]);
+    const float v2l = wl[MOM] * wl[MOM] + wl[MOM + 1] * wl[MOM + 1] + wl[MOM + 2] * wl[MOM + 2];
+    const float v2r = wr[MOM] * wr[MOM] + wr[MOM + 1] * wr[MOM + 1] + wr[MOM + 2] * wr[MOM + 2];
+    const float el = wl[ENE] * cv_kb + wl[RHO] * v2l * 0.5f;
+    const float er = wr[ENE] * cv_kb + wr[RHO] * v2r * 0.5f;
+    const float cl = fsq(gam * pmax(wl[ENE] * irl, 0.f));
+    const float cr = fsq(gam * pmax(wr[ENE] * irr, 0.f));
+    const float sdl = fsq(wl[RHO]), sdr = fsq(wr[RHO]);
+    const float isd = frcp(sdl + sdr);
+    float wroe[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) wroe[n] = (sdl * wl[MOM + n] + sdr * wr[MOM + n]) * isd;
+    // (e + p) / sqrt(rho) = (e + p) * sqrt(rho) / rho
+    const float hroe = isd * ((el + wl[ENE]) * sdl * irl + (er + wr[ENE]) * sdr * irr);
+    // gamma - 1 of the Roe state: both sides share gamma, so the weighted
+    // mean of 1 / (gamma - 1) is cv_kb itself
+    const float gam1_roe = frcp(isd * (sdl * cv_kb + sdr * cv_kb));
+    const float v2roe = wroe[0] * wroe[0] + wroe[1] * wroe[1] + wroe[2] * wroe[2];
+    const float q = hroe - 0.5f * v2roe;
+    const float a = q <= 0.f ? 0.f : fsq(gam1_roe * q);
+    const float al = pmin(wroe[0] - a, wl[MOM] - cl);
+    const float ar = pmax(wroe[0] + a, wr[MOM] + cr);
+    const float bp = ar > 0.f ? ar : 0.f;
+    const float bm = al < 0.f ? al : 0.f;
+    const float vxl = wl[MOM] - al, vxr = wr[MOM] - ar;
+    const float tl = wl[ENE] + vxl * wl[RHO] * wl[MOM];
+    const float tr = wr[ENE] + vxr * wr[RHO] * wr[MOM];
+    const float ml = wl[RHO] * vxl, mr = -wr[RHO] * vxr;
+    const float f_max = FLT_MAX / 1e1f, f_min = FLT_MIN * 1e1f;
+    float cp, am;
+    const float msum = ml + mr;
+    if (kfabs(msum) < f_min) {
+        cp = 0.f;
+        am = 0.f;
+    } else {
+        const float ims = frcp(msum);
+        am = (tl - tr) * ims;
+        if ((tr != 0.f && kfabs(ml) > f_max * frcp(kfabs(tr))) ||
+            (tl != 0.f && kfabs(mr) > f_max * frcp(kfabs(tl)))) {
+            const float mli = (float)vsgn(ml) * frcp(pmax(kfabs(ml), f_min));
+            const float mri = (float)vsgn(mr) * frcp(pmax(kfabs(mr), f_min));
+            cp = pmax((mri * tr + mli * tl) * frcp(mli + mri), 0.f);
+        } else {
+            cp = pmax((ml * tr + mr * tl) * ims, 0.f);
+        }
+    }
+    const float d_am_bm = pmax(kfabs(am - bm), f_min) * (am > bm ? 1.f : -1.f);
+    const float d_am_bp = pmax(kfabs(am - bp), f_min) * (am > bp ? 1.f : -1.f);
+    const float i_bm = frcp(d_am_bm), i_bp = frcp(d_am_bp);
+    const float sl = am >= 0.f ? am * i_bm : 0.f;
+    const float sr = am < 0.f ? am * i_bp : 0.f;
+    const float sm = am > 0.f ? -bm * i_bm : -bp * i_bp;
+    const float vl = wl[MOM] - bm, vr = wr[MOM] - bp;
+    flx[RHO] = sl * (wl[RHO] * vl) + sr * (wr[RHO] * vr);
+    flx[ENE] = sl * (wl[ENE] * wl[MOM] + el * vl) + sr * (wr[ENE] * wr[MOM] + er * vr);
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        float fl = wl[RHO] * wl[MOM + n] * vl, fr = wr[RHO] * wr[MOM + n] * vr;
+        if (n == 0) {
+            fl += wl[ENE];
+            fr += wr[ENE];
+        }
+        flx[MOM + n] = sl * fl + sr * fr;
+    }
+    flx[MOM] += sm * cp;
+    flx[ENE] += sm * cp * am;
+}
+
 }  // namespace kr

@@ -18,7 +18,8 @@
 // instructions, so every face flux has one value and the update conserves
 // exactly.  The per-face arithmetic and the order of the update sums are
 // those of the staged kernels (k_flux, k_update), so the FP64 form equals
-// them, and the oracle, bit for bit.  hydro/integrator/integrator.h:56-68,
+// them, and the oracle, bit for bit, and the mixed form (hllc_fast,
+// plm_fast) equals the staged mixed form bit for bit.  hydro/integrator/integrator.h:56-68,
 // 156-207, 241-265; hydro/reconstruct/plm.h:116-153; hydro/riemann/
 // hllc.h:32-193; rk2.h:20-41.
 #pragma once
@@ -146,7 +147,10 @@ __global__ void __launch_bounds__(NT) k_sweep_hyd(MeshView<FT> m, int step) {
                     }
                     rot<2>(wl + MOM);
                     rot<2>(wr + MOM);
-                    hllc<FT>(flx, wl, wr, gam);
+                    if constexpr (std::is_same_v<FT, float>)
+                        hllc_fast(flx, wl, wr, gam);
+                    else
+                        hllc<FT>(flx, wl, wr, gam);
                     rot<1>(flx + MOM);
                     if (P >= 0) {
                         // z part of cell P: faces P (previous) and p = P + 1
@@ -209,7 +213,10 @@ __global__ void __launch_bounds__(NT) k_sweep_hyd(MeshView<FT> m, int step) {
                             wr[k] = q00 + (fr ? (FT)0 : -tr);
                         }
                     }
-                    hllc<FT>(flx, wl, wr, gam);
+                    if constexpr (std::is_same_v<FT, float>)
+                        hllc_fast(flx, wl, wr, gam);
+                    else
+                        hllc<FT>(flx, wl, wr, gam);
                     sF[RHO][it] = flx[RHO];
                     sF[ENE][it] = flx[ENE];
 #pragma unroll

@@ -444,7 +444,10 @@ __device__ __forceinline__ void face_flux(FT *ql, FT *qr, FT *flx, FT gam) {
         flx[BCC + (AXIS + 2) % 3] = e1;
         flx[BCC + AXIS] = 0;
     } else {
-        hllc<FT>(flx, ql, qr, gam);
+        if constexpr (std::is_same_v<FT, float>)
+            hllc_fast(flx, ql, qr, gam);
+        else
+            hllc<FT>(flx, ql, qr, gam);
         rot<3 - AXIS>(flx + MOM);
     }
 }
