@@ -202,6 +202,18 @@ k_sweep(MeshView<FT> m, int step) {
     // of every plane).
     __shared__ int sNbr[27];
     if (tid < 27) sNbr[tid] = g.wnbr[tid];
+    // Per-block factors of the update and the CT, formed once per launch
+    // (the same expressions as at each use, so the same values): CT factor
+    // -dt / sfc_a, flux factor sfc_a * dt / vol, edge length dx_a.  Kept in
+    // LDS, not registers, and read where used by the FP32 flux form; the
+    // FP64 form forms them at each use (its code is faster that way).
+    __shared__ double sGeo[9];
+    if (tid < 3) {
+        const double sfc = g.sfc[tid];
+        sGeo[tid] = sfc > 0 ? (double)(-dt / sfc) : (double)(0);
+        sGeo[3 + tid] = sfc * (dt / g.vol);
+        sGeo[6 + tid] = g.dx[tid];
+    }
     __syncthreads();
 
     // ---- primitive source of halo column (cx, cy) at plane z
@@ -325,7 +337,9 @@ k_sweep(MeshView<FT> m, int step) {
     const bool xedge = xlast && et >= 0 && et < TY;
     const bool yedge = ylast && et >= TY && et < TY + TX;
 
+    constexpr bool F32 = std::is_same_v<FT, float>;
     const double dt_dv = dt / g.vol;
+
 
     issue_w(-2, ring3(-2));
     for (int L = -2; L <= nz + 1; ++L) {
@@ -397,7 +411,7 @@ k_sweep(MeshView<FT> m, int step) {
                     } else if (zint) {
                         // z part of the cell finished now (plane p-1 = P)
                         if (P >= 0) {
-                            const FT fa = (FT)(g.sfc[2] * dt_dv);
+                            const FT fa = F32 ? (FT)sGeo[3 + 2] : (FT)(g.sfc[2] * dt_dv);
                             const size_t cc = cidx(m.n, x0 + cx_, y0 + cy_, P);
                             double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + cc;
 #pragma unroll
@@ -609,15 +623,21 @@ k_sweep(MeshView<FT> m, int step) {
             };
             // CT circulation of one face (mhd/integrator/integrator.h:429-457)
             auto ct = [&](double e1, double d1, double e2, double d2, double e3, double e4,
-                          double sfc) {
+                          int a) {
                 double db = e1 * d1;
                 db -= e2 * d2;
                 db += e3 * d2;
                 db -= e4 * d1;
-                db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
+                if constexpr (F32) {
+                    db *= sGeo[a];              // -dt / sfc, or 0 for a zero-area face
+                } else {
+                    const double sfc = g.sfc[a];
+                    db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
+                }
                 return db;
             };
-            const double dx = g.dx[0], dy = g.dx[1], dz = g.dx[2];
+            const double dx = F32 ? sGeo[6] : g.dx[0], dy = F32 ? sGeo[7] : g.dx[1],
+                         dz = F32 ? sGeo[8] : g.dx[2];
             if (zint) {
                 const int x = cx_, y = cy_;
                 // x + y partial of plane q (cells), finished next iteration
@@ -651,7 +671,7 @@ k_sweep(MeshView<FT> m, int step) {
                                 uu[v] += res;
                             }
                         } else {
-                            const float fa = (float)(g.sfc[a] * dt_dv);
+                            const float fa = (float)sGeo[3 + a];
 #pragma unroll
                             for (int v = 0; v < 5; ++v) {
                                 double res(0);
@@ -669,29 +689,29 @@ k_sweep(MeshView<FT> m, int step) {
                 // z face q
                 set_face(2, x, y, q,
                          ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
-                            sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], g.sfc[2]));
+                            sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
                 if (P >= 0) {
                     // x face (x, y, P): a1 = y, a2 = z
                     set_face(0, x, y, P,
                              ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
-                                sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], g.sfc[0]));
+                                sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
                     // y face (x, y, P): a1 = z, a2 = x
                     set_face(1, x, y, P,
                              ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
-                                sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], g.sfc[1]));
+                                sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
                 }
             }
             if (P >= 0 && xedge) {
                 const int x = TX, y = et;
                 set_face(0, x, y, P,
                          ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
-                            sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], g.sfc[0]));
+                            sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
             }
             if (P >= 0 && yedge) {
                 const int x = et - TY, y = TY;
                 set_face(1, x, y, P,
                          ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
-                            sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], g.sfc[1]));
+                            sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
             }
         }
     }
