@@ -53,6 +53,11 @@ __global__ void __launch_bounds__(NT) k_sweep_hyd(MeshView<FT> m, int step) {
     __shared__ FT sF[5][NF];            // in-plane face fluxes of one plane, mesh frame
     __shared__ int sNbr[27];
     if (tid < 27) sNbr[tid] = g.wnbr[tid];
+    // flux factors sfc_a * dt / vol of the FP32 form, formed once per launch
+    // in LDS (the expression of each use, so the same values)
+    constexpr bool F32 = std::is_same_v<FT, float>;
+    __shared__ double sFa[3];
+    if (tid < 3) sFa[tid] = g.sfc[tid] * (dt / g.vol);
     __syncthreads();
 
     auto fxi = [](int x, int y) { return y * (TX + 1) + x; };
@@ -154,7 +159,7 @@ __global__ void __launch_bounds__(NT) k_sweep_hyd(MeshView<FT> m, int step) {
                     rot<1>(flx + MOM);
                     if (P >= 0) {
                         // z part of cell P: faces P (previous) and p = P + 1
-                        const FT fa = (FT)(g.sfc[2] * dt_dv);
+                        const FT fa = F32 ? (FT)sFa[2] : (FT)(g.sfc[2] * dt_dv);
                         const size_t cc = cidx(m.n, x0 + cx, y0 + cy, P);
                         double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + cc;
 #pragma unroll
@@ -257,7 +262,7 @@ __global__ void __launch_bounds__(NT) k_sweep_hyd(MeshView<FT> m, int step) {
                         uu[v] += res;
                     }
                 } else {
-                    const float fa = (float)(g.sfc[a] * dt_dv);
+                    const float fa = (float)sFa[a];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) {
                         double res(0);
