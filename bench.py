@@ -6,7 +6,10 @@ A "step" is one Heun cycle (two RK substeps) over every cell; cell-updates/s
 = cells x cycles / wall seconds, the reference's Speed definition
 (mesh/cycle/cycle.cpp:184-190).  The state is resident in HBM before the
 timed region; the timed region is K cycles replayed from the captured
-hipGraph, bracketed by device synchronisation.
+hipGraph, bracketed by device synchronisation.  Before it, after the W
+warmup cycles, further untimed cycles run until the tile autotuner has
+timed every candidate and the graph is instantiated (warmup_cycles_run
+reports the total).
 
 Both precision forms run: the plain FP64 form and the mixed form (FP32
 reconstruction / Riemann / EMF, FP64 conserved state, CT, energy and dt).
@@ -14,11 +17,13 @@ reconstruction / Riemann / EMF, FP64 conserved state, CT, energy and dt).
 reported beside it.
 
 accuracy: the same recipe at beta = 1 and beta = 0.01, both forms run to
-t = 0.1 (t_lim top-up); relative L1 of rho, E and B of the mixed form
-against the FP64 form (which equals the CPU oracle bit for bit on every
-case the parity tests run), and max |div B|.  At beta = 0.01 also the L1
+t = 0.1 (t_lim top-up), and beta = 0.01 once more to t = 0.01; relative L1
+of rho, E and B of the mixed form against the FP64 form (which equals the
+CPU oracle bit for bit on every case the parity tests run), and max
+|div B| (absolute, and times dx over max |B|).  At beta = 0.01 also the L1
 distance that a 1e-12 relative perturbation of the initial density reaches
-in the FP64 form alone: the problem's own sensitivity floor.
+in the FP64 form alone: the problem's own sensitivity floor, which by
+t = 0.1 has grown past the mixed form's distance and at t = 0.01 has not.
 
 hydro: the hydro module (PLM + HLLC + RK2) on the same 256^3 recipe without
 the face field, 512 blocks of 32^3, both forms (fused hydro kernel).
@@ -43,7 +48,8 @@ measured here; stream_copy_gbs: a plain 1 GiB device copy timed on the
 same GPU, the rate a streaming kernel reaches there (frac_of_stream_copy
 puts the dominant kernel against it).  cpu_baseline: the CPU
 oracle (FP64 C++ restatement of the reference algorithm, "port") on a
-bounded sample of the same recipe.
+bounded sample of the same recipe: 128^3 as 512 blocks of 16^3, on the
+cores the process may use (at most 16), with nproc and the CPU model.
 """
 import argparse
 import json
@@ -110,6 +116,24 @@ def sync_all(ctx):
     ctx.lib.kr_sync(ctx.h)
 
 
+def settle(ctx, max_cycles=64):
+    """Untimed cycles until the tile autotuner has timed every candidate
+    and the cycle's hipGraph is instantiated, so that a timed kr_advance
+    afterwards is graph replays only.  Returns the cycles it ran."""
+    extra = 0
+    while extra < max_cycles:
+        st = ctx.stats()
+        if st["tune_pending"] == 0 and st["graph_ready"]:
+            break
+        if ctx.advance(1) == 0:
+            break                               # every time group at t_lim
+        extra += 1
+    st = ctx.stats()
+    if st["tune_pending"] or not st["graph_ready"]:
+        raise RuntimeError(f"autotuner / graph not settled after {extra} cycles: {st}")
+    return extra
+
+
 def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="auto",
              tile=(0, 0, 0)):
     import kratos_amd
@@ -117,6 +141,7 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
     ctx.upload(u, b)
     if warmup:
         ctx.advance(warmup)
+    extra = settle(ctx)
     sync_all(ctx)
     t0 = time.perf_counter()
     done = ctx.advance(steps)
@@ -125,6 +150,7 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
     st = ctx.stats()
     prof = ctx.profile(profile_cycles) if profile_cycles else None
     return ctx, {"wall": wall, "cycles": done, "gpu_ms": st["gpu_ms"], "prof": prof,
+                 "warmup_cycles_run": warmup + extra,
                  "n_cells": st["n_cells"], "sweep_tile": st["sweep_tile"],
                  "tune_ms": [round(t, 4) for t in st["tune_ms"]]}
 
@@ -179,7 +205,7 @@ def ensemble_leg(n_box, n, t_lim, path, nb=None):
         ctx = kratos_amd.Context(spec, phys, form, path=path, members=n_box)
         for m, (u, b) in enumerate(ics):
             ctx.upload(u, b, member=m)
-        ctx.advance(8)                      # untimed: autotuner cycles and graph capture
+        settle(ctx)                         # untimed: autotuner cycles and graph capture
         sync_all(ctx)
         c0 = [ctx.time(member=m)[2] for m in range(n_box)]
         t0 = time.perf_counter()
@@ -223,6 +249,7 @@ def hydro_leg(n, nb, steps, warmup, path):
         ctx = kratos_amd.Context(spec, phys, form, path=path)
         ctx.upload(u, None)
         ctx.advance(warmup)
+        settle(ctx)
         sync_all(ctx)
         t0 = time.perf_counter()
         done = ctx.advance(steps)
@@ -271,6 +298,7 @@ def smr_leg(n_root, nb, steps, warmup, cycles_acc):
         states[form] = ctx.download_blocks()
         out[f"divb_max_{form}"] = ctx.divb_max()
         ctx.advance(warmup)
+        settle(ctx)
         sync_all(ctx)
         t0 = time.perf_counter()
         done = ctx.advance(steps)
@@ -299,13 +327,28 @@ def smr_leg(n_root, nb, steps, warmup, cycles_acc):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(seconds_target=15.0, threads=None):
-    """Oracle on a bounded sample: 64^3 cells of the same recipe (8 blocks of 32^3)."""
+    """Oracle on a bounded sample: 128^3 cells of the same recipe as 512
+    blocks of 16^3 (the oracle spreads blocks over its threads, so this
+    keeps every thread busy), on every core the process may use, capped at
+    16 (the host share of one GPU on the measurement machine)."""
     import kratos_amd
     import oracle
-    threads = threads or min(16, os.cpu_count() or 1)
-    n = 64
-    spec = kratos_amd.MeshSpec((n, n, n), (32, 32, 32))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = threads or min(16, avail or 1)
+    n = 128
+    spec = kratos_amd.MeshSpec((n, n, n), (16, 16, 16))
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
     u, b, _ = make_problem(n, seed=0)
     o = oracle.OracleMesh(spec, phys, n_threads=threads)
@@ -316,9 +359,11 @@ def cpu_baseline(seconds_target=15.0, threads=None):
         o.advance(1)
         cycles += 1
     wall = time.perf_counter() - t0
+    assert spec.n_blocks >= 4 * threads
     return {"value": n ** 3 * cycles / wall, "unit": "cell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"64^3 MHD turbulence (8 blocks of 32^3), {cycles} cycles, "
+            "kind": "port", "nproc": os.cpu_count(), "cpus_available": avail,
+            "cpu_model": cpu_model(),
+            "sample": f"128^3 MHD turbulence (512 blocks of 16^3), {cycles} cycles, "
                       f"{wall:.1f} s, FP64 C++ oracle on {threads} threads"}
 
 
@@ -389,6 +434,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--t-acc", type=float, default=0.1)
+    ap.add_argument("--t-acc-low", type=float, default=0.01,
+                    help="second, early time of the beta = 0.01 accuracy run (0: skip)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-n", type=int, default=64)
@@ -496,10 +543,15 @@ def main():
 
     acc = []
     if not args.no_accuracy:
-        for beta in (1.0, 0.01):
-            a = accuracy_leg(n, nb, beta, args.t_acc, args.path, sensitivity=beta < 0.1)
+        runs = [(1.0, args.t_acc), (0.01, args.t_acc)]
+        if args.t_acc_low > 0:
+            # low beta at an early time, where the problem's own sensitivity
+            # floor (the perturbed FP64 run) is still far below the 1e-5 bar
+            runs.append((0.01, args.t_acc_low))
+        for beta, t_end in runs:
+            a = accuracy_leg(n, nb, beta, t_end, args.path, sensitivity=beta < 0.1)
             acc.append(a)
-            print(f"[bench] accuracy beta={beta}: {a}", file=sys.stderr, flush=True)
+            print(f"[bench] accuracy beta={beta} t={t_end}: {a}", file=sys.stderr, flush=True)
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg(args.ens_boxes, args.ens_n, args.t_ens, args.path)
@@ -541,6 +593,7 @@ def main():
                       "ms_per_cycle": v["wall"] / max(v["cycles"], 1) * 1e3,
                       "divb_max": v["divb"], "divb_rel": v["divb_rel"],
                       "sweep_tile": v["sweep_tile"], "autotune_ms": v["tune_ms"],
+                      "warmup_cycles_run": v["warmup_cycles_run"],
                       "kernel_ms": v["prof"]["kernel_ms"] if v["prof"] else None}
                   for f, v in res.items()},
         "speedup_mixed_over_fp64": (res["mixed"]["rate"] / res["fp64"]["rate"]

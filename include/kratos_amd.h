@@ -106,6 +106,11 @@ typedef struct kr_stats {
                               per SIMD asked (zeros: staged path)           */
     float  tune_ms[8];     /* autotuner: mean stage-kernel time per variant
                               over its tuning cycle (0: not timed)          */
+    int    tune_pending;   /* variants the autotuner has still to time: the
+                              next that many cycles run eagerly, one
+                              variant each                                  */
+    int    graph_ready;    /* 1 once the cycle's hipGraph is instantiated
+                              (kr_advance then only replays it)            */
 } kr_stats;
 
 enum kr_kernel_kind {
@@ -183,6 +188,9 @@ int kr_get_stats(kr_ctx *ctx, kr_stats *out);
 /* Block until all queued device work of the context has finished.       */
 int kr_sync(kr_ctx *ctx);
 
+/* Message of the last failed call on ctx; with ctx == NULL, the reason the
+ * last kr_create on this thread returned no context (the reference throws
+ * std::runtime_error with such a text, device/hip/hip.cpp:8-16).         */
 const char *kr_last_error(kr_ctx *ctx);
 void kr_destroy(kr_ctx *ctx);
 

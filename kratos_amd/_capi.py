@@ -62,6 +62,8 @@ class KrStats(C.Structure):
         ("alg_bytes_per_cell_update", C.c_double),
         ("sweep_tile", C.c_int * 4),
         ("tune_ms", C.c_float * 8),
+        ("tune_pending", C.c_int),
+        ("graph_ready", C.c_int),
     ]
 
 
@@ -110,8 +112,8 @@ def load():
 
 def check(rc, ctx=None, what=""):
     if rc != 0:
-        msg = ""
-        if ctx:
-            m = load().kr_last_error(ctx)
-            msg = m.decode() if m else ""
+        # with no context, kr_last_error(NULL) gives the reason of the last
+        # kr_create on this thread
+        m = load().kr_last_error(ctx if ctx else None)
+        msg = m.decode() if m else ""
         raise KratosError(f"{what} failed ({rc}): {msg}")

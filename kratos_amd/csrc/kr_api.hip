@@ -832,6 +832,8 @@ struct Launcher {
 
 void fill_tile_stats(kr_ctx *c) {
     for (int k = 0; k < 4; ++k) c->stats.sweep_tile[k] = 0;
+    c->stats.tune_pending = (c->fused && !c->tuned) ? (int)c->candidates.size() - c->tune_next : 0;
+    c->stats.graph_ready = c->graph != nullptr;
     if (!c->fused) return;
     const SweepVariant &sv = kSweepVariants[c->variant];
     c->stats.sweep_tile[0] = sv.tx;
@@ -871,6 +873,10 @@ void tune_cycle(kr_ctx *c) {
         c->tuned = true;
     }
 }
+
+// Message of the last kr_create that returned no context on this thread
+// (kr_last_error(NULL) returns it).
+thread_local std::string g_create_err;
 
 int fail(kr_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -922,7 +928,11 @@ bool all_done(const std::vector<TimeState> &v) {
 extern "C" {
 
 int kr_create(const kr_config *cfg, kr_ctx **out) {
-    if (!cfg || !out) return KR_EINVAL;
+    g_create_err.clear();
+    if (!cfg || !out) {
+        g_create_err = "kr_create: null config or output pointer";
+        return KR_EINVAL;
+    }
     *out = nullptr;
     auto *c = new (std::nothrow) kr_ctx();
     if (!c) return KR_ENOMEM;
@@ -953,6 +963,7 @@ int kr_create(const kr_config *cfg, kr_ctx **out) {
     for (int s = 0; s < 6 && why.empty(); ++s)
         if (cfg->bnd[s] < KR_PER || cfg->bnd[s] > KR_FIX) why = "unknown boundary kind";
     if (!why.empty()) {
+        g_create_err = "kr_create: " + why;
         delete c;
         return KR_EINVAL;
     }
@@ -966,7 +977,7 @@ int kr_create(const kr_config *cfg, kr_ctx **out) {
         return KR_OK;
     });
     if (rc != KR_OK) {
-        fprintf(stderr, "kr_create: %s\n", c->err.c_str());
+        g_create_err = "kr_create: " + c->err;
         delete c;
         return rc;
     }
@@ -1256,7 +1267,10 @@ int kr_sync(kr_ctx *c) {
     });
 }
 
-const char *kr_last_error(kr_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *kr_last_error(kr_ctx *c) {
+    if (c) return c->err.c_str();
+    return g_create_err.empty() ? "null context" : g_create_err.c_str();
+}
 
 void kr_destroy(kr_ctx *c) {
     if (!c) return;

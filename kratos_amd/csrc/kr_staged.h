@@ -239,10 +239,6 @@ __device__ __forceinline__ double load_bf(const MeshView<FT> &m, int b, int slot
     return m.BF[(((size_t)blk * 2 + slot) * 3 + c) * m.nface + fidx(m.n, p[0], p[1], p[2])];
 }
 
-// ------------------------------------------------------------- time state
-__global__ void k_cycle_begin(TimeState *ts);
-__global__ void k_dt_finalize(TimeState *ts, const double *part, int npart);
-
 // Wave64 + workgroup min reduction of a double; lane 0 of the workgroup
 // returns the result.
 __device__ __forceinline__ double block_min(double v) {

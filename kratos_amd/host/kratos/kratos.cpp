@@ -14,7 +14,7 @@ namespace mesh {
 
 void check(int rc, kr_ctx *ctx, const char *what) {
     if (rc == KR_OK) return;
-    const char *m = ctx ? kr_last_error(ctx) : nullptr;
+    const char *m = kr_last_error(ctx);     // NULL: the last kr_create's reason
     throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) +
                              "): " + (m ? m : ""));
 }

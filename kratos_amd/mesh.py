@@ -36,6 +36,58 @@ def bf_host_size(n):
     return N[0] * N[1] * N[2] * 3
 
 
+def _divb_cells(bfs, dx):
+    """|div B| per cell from face fields bx (nz, ny, nx+1), by (nz, ny+1,
+    nx), bz (nz+1, ny, nx); a thin axis (one face, both sides of the cell)
+    contributes nothing."""
+    d = 0.0
+    for a, f in enumerate(bfs):
+        ax = 2 - a                                   # arrays are [z, y, x]
+        if f.shape[ax] < 2:
+            continue
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        cell = [min(g.shape[k] for g in bfs) for k in range(3)]
+        cut = tuple(slice(0, cell[k]) for k in range(3))
+        d = d + (f[tuple(hi)] - f[tuple(lo)])[cut] / dx[a]
+    return np.abs(d)
+
+
+def divb_rel(bfs, dx):
+    """The div B bar in dimensionless form: max over cells of |div B| times
+    the smallest cell width, over the largest face |B|.  This is the net
+    magnetic flux out of a cell relative to the largest face flux, so a
+    value near 1e-16 is one rounding; constrained transport keeps it at
+    round-off whatever the field strength and resolution, where an
+    absolute |div B| scales with |B| / dx."""
+    bmax = max(float(np.abs(f).max()) for f in bfs)
+    d = _divb_cells(bfs, dx)
+    return float(np.max(d)) * min(dx) / bmax if bmax > 0 else float(np.max(d))
+
+
+def divb_rel_blocks(spec, blocks):
+    """divb_rel over the per-block host arrays of `spec` (any block mesh):
+    each block's cells with its own widths, over the largest face |B| of
+    the mesh."""
+    n = spec.n_cell_block
+    nf = bf_faces(n)
+    dmax, bmax = 0.0, 0.0
+    for b, (_, ba) in enumerate(blocks):
+        _, dx = spec.geometry(b)
+        f = np.asarray(ba).reshape(bf_host_dims(n)[::-1] + (3,))
+        faces = []
+        for c in range(3):
+            sl = [slice(2, n[a] + 2) for a in range(3)]
+            sl[c] = slice(2, nf[c] + 2)
+            faces.append(f[sl[2], sl[1], sl[0], c])
+        bmax = max(bmax, max(float(np.abs(x).max()) for x in faces))
+        act = [dx[a] for a in range(3) if n[a] > 1]
+        dmax = max(dmax, float(np.max(_divb_cells(faces, dx))) * min(act))
+    return dmax / bmax if bmax > 0 else dmax
+
+
 @dataclass
 class MeshSpec:
     n_cell_global: tuple
@@ -535,4 +587,6 @@ class Context:
             "alg_bytes_per_cell_update": s.alg_bytes_per_cell_update,
             "sweep_tile": tuple(s.sweep_tile),
             "tune_ms": tuple(s.tune_ms),
+            "tune_pending": s.tune_pending,
+            "graph_ready": bool(s.graph_ready),
         }

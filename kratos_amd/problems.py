@@ -210,6 +210,44 @@ def orszag_tang(N=(256, 256, 1), gamma=5.0 / 3.0):
     return u, (bx, by, bz)
 
 
+def field_loop(N=(128, 64, 1), gamma=5.0 / 3.0, vel=(2.0, 1.0, 0.0), a0=1e-3, r0=0.3,
+               rho0=1.0, p0=1.0, x_min=(-1.0, -0.5, -0.5), x_max=(1.0, 0.5, 0.5)):
+    """Field-loop advection (Gardiner & Stone 2005, JCP 205, 509, sec. 5.1):
+    a weak loop of field lines, B = curl(A_z e_z) with A_z = a0 (r0 - r) for
+    r < r0 (r in the x-y plane), carried by the uniform flow `vel` through
+    the periodic box at rho0, p0.  A_z sits on the z edges, so the face
+    field has div B = 0 exactly and Bz = 0; the loop is uniform along z
+    (N[2] = 1 gives the 2-D problem).  With vel = (2, 1, .) on the default
+    2 x 1 box the loop returns to its start at t = 1."""
+    Nx, Ny, Nz = N
+    dx = (x_max[0] - x_min[0]) / Nx
+    dy = (x_max[1] - x_min[1]) / Ny
+    xf = x_min[0] + np.arange(Nx + 1) * dx
+    yf = x_min[1] + np.arange(Ny + 1) * dy
+    r = np.sqrt(xf[None, :] ** 2 + yf[:, None] ** 2)
+    az = np.where(r < r0, a0 * (r0 - r), 0.0)                # (Ny+1, Nx+1)
+    bx2 = (az[1:, :] - az[:-1, :]) / dy                       # (Ny, Nx+1)
+    by2 = -(az[:, 1:] - az[:, :-1]) / dx                      # (Ny+1, Nx)
+    bx = np.broadcast_to(bx2[None], (Nz, Ny, Nx + 1)).copy()
+    by = np.broadcast_to(by2[None], (Nz, Ny + 1, Nx)).copy()
+    bz = np.zeros((Nz + 1, Ny, Nx))
+    rho = np.full((Nz, Ny, Nx), rho0)
+    pre = np.full((Nz, Ny, Nx), p0)
+    v = [np.full((Nz, Ny, Nx), vel[a]) for a in range(3)]
+    u = np.zeros((5, Nz, Ny, Nx))
+    u[0] = rho
+    for a in range(3):
+        u[2 + a] = rho * v[a]
+    u[1] = total_energy(rho, pre, v, cell_centred_b(bx, by, bz), gamma)
+    return u, (bx, by, bz)
+
+
+def magnetic_energy(bfs):
+    """Sum over cells of |Bcc|^2 / 2 (per unit cell volume)."""
+    bcc = cell_centred_b(*bfs)
+    return 0.5 * float(sum(np.sum(c * c) for c in bcc))
+
+
 def brio_wu(N=(512, 1, 1), gamma=2.0, x_split=0.5, bx0=0.75):
     """Brio & Wu (1988) MHD shock tube along x on [0, 1]: (rho, p, By) =
     (1, 1, 1) for x < x_split and (0.125, 0.1, -1) beyond, Bx = 0.75,

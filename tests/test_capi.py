@@ -57,7 +57,7 @@ int main(void) {
     F(kr_block_desc, xf0); F(kr_block_desc, dx0); F(kr_block_desc, nbr);
     F(kr_stats, n_cells); F(kr_stats, kernel_ms); F(kr_stats, kernel_calls);
     F(kr_stats, alg_bytes_per_cell_update); F(kr_stats, sweep_tile); F(kr_stats, tune_ms);
-    F(kr_block_desc, group);
+    F(kr_block_desc, group); F(kr_stats, tune_pending); F(kr_stats, graph_ready);
     return 0;
 }
 """
@@ -90,7 +90,20 @@ def test_create_rejects_bad_config_without_device():
     h = C.c_void_p()
     rc = lib.kr_create(C.byref(cfg), C.byref(h))
     assert rc == -1 and not h.value
+    # the reason comes back through kr_last_error(NULL)
+    assert lib.kr_last_error(None).decode() == "kr_create: unknown model"
+    cfg.model = 1
+    cfg.cfl = 0.0
+    assert lib.kr_create(C.byref(cfg), C.byref(h)) == -1
+    assert "cfl" in lib.kr_last_error(None).decode()
+    cfg.cfl = 0.3
+    cfg.bnd[:] = [0, 1, 0, 0, 0, 0]
+    assert lib.kr_create(C.byref(cfg), C.byref(h)) == -1
+    assert "periodic" in lib.kr_last_error(None).decode()
+    with pytest.raises(_capi.KratosError, match="periodic must be set on both sides"):
+        _capi.check(-1, None, "kr_create")
     assert lib.kr_create(None, C.byref(h)) == -1
+    assert "null" in lib.kr_last_error(None).decode()
     lib.kr_destroy(None)                        # no-op on a null handle
 
 

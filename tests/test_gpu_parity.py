@@ -1,6 +1,7 @@
 """GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
 same seeded states.  FP64 form: relative L1 <= 1e-12 per field (the bar of
-the plain FP64 form); mixed form: density L1 <= 1e-5 relative.
+the plain FP64 form); mixed form: density L1 <= 1e-5 relative.  div B, in
+both forms: max |div B| dx / max |B| <= 1e-12 (kratos_amd.mesh.divb_rel).
 """
 import numpy as np
 import pytest
@@ -8,6 +9,7 @@ import pytest
 import kratos_amd
 import oracle
 from kratos_amd import problems
+from kratos_amd.mesh import divb_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +20,10 @@ def rel_l1(a, b):
     return d / s if s > 0 else d
 
 
+def spec_dx(spec):
+    return tuple((spec.x_max[a] - spec.x_min[a]) / spec.n_cell_global[a] for a in range(3))
+
+
 def run_both(spec, phys, u, b, n_cycles, precision="fp64", path="auto"):
     print(f"\n[case] N={spec.n_cell_global} blk={spec.n_cell_block} mhd={phys.mhd} "
           f"prec={precision} path={path} cycles={n_cycles}", flush=True)
@@ -26,8 +32,8 @@ def run_both(spec, phys, u, b, n_cycles, precision="fp64", path="auto"):
     done = ctx.advance(n_cycles)
     gu, gb = ctx.download()
     gt = ctx.time()
-    if phys.mhd:
-        assert ctx.divb_max() < 1e-11          # both forms: CT at round-off
+    if phys.mhd:                               # both forms: CT at round-off
+        assert divb_rel(gb, spec_dx(spec)) <= 1e-12
     ctx.close()
     ref = oracle.OracleMesh(spec, phys)
     ref.upload(u, b)
@@ -199,9 +205,8 @@ def test_fused_tile_variants(gpu_lib, tile, precision):
     gu, gb = ctx.download()
     # CT keeps div B at round-off in both forms; a face that two tiles both
     # compute with different rounding would show up here (about 1e-6)
-    divb = ctx.divb_max()
     ctx.close()
-    assert divb < 1e-11
+    assert divb_rel(gb, spec_dx(spec)) <= 1e-12
     if precision == "fp64":
         np.testing.assert_array_equal(gu, ou)
         for x, y in zip(gb, ob):
@@ -251,7 +256,6 @@ def test_mhd_turbulence_128_fp64_100_cycles(gpu_lib):
     g10 = ctx.download()
     assert ctx.advance(90) == 90
     gu, gb = ctx.download()
-    divb = ctx.divb_max()
     ctx.close()
     ref = oracle.OracleMesh(spec, phys, n_threads=16)
     ref.upload(u, b)
@@ -263,7 +267,7 @@ def test_mhd_turbulence_128_fp64_100_cycles(gpu_lib):
     np.testing.assert_array_equal(gu, ou)
     for x, y in zip(gb, ob):
         np.testing.assert_array_equal(x, y)
-    assert divb < 1e-11
+    assert divb_rel(gb, spec_dx(spec)) <= 1e-12
 
 
 @pytest.mark.parametrize("bnd", [("out", "out", "ref", "ref", "fre", "fre"),

@@ -9,13 +9,19 @@ term, a wrong sign or a swapped operand fails at least one of them:
   extrema flattened, the van-Leer harmonic slope against a direct formula.
 * HLLC (hydro/riemann/hllc.h:76-193): consistency F(U, U) = f(U), supersonic
   upwinding returns the physical flux of the upwind state, a stationary
-  contact carries only the pressure, mirror symmetry.
+  contact carries only the pressure, mirror symmetry.  Its FLT-constant
+  guards (hllc.h:140-165), which stay in FP64: the overflow branch by the
+  flux's scale homogeneity (a problem scaled by 1e20 takes that branch and
+  must give 1e20 times the unit-scale flux), the degenerate branch by the
+  zero flux of a cold gas at rest.
 * HLLD (mhd/riemann/hlld.h:81-298): consistency, supersonic upwinding, an
   isolated stationary contact, mirror symmetry, and the face-EMF rewrite
   (hlld.h:42-46) against E = -v x B for a uniform state.
 * c2p (hydro/eos/adiabatic.h:159-206): primitive -> conserved -> primitive
   round trip, density / pressure floors and the healed energy.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -114,6 +120,66 @@ def test_hllc_mirror_symmetry():
         g = oracle.hllc(ml, mr, 1.4)
         # mass, energy and transverse momentum flip sign; normal momentum does not
         np.testing.assert_allclose(g, [-f[0], -f[1], f[2], -f[3], -f[4]], rtol=1e-12, atol=1e-13)
+
+
+def _hllc_gate(wl, wr, gamma):
+    """ml, mr, tl, tr of hllc.h:122-131 (the quantities its FLT-constant
+    guards test), from the wave-speed bounds al, ar of hllc.h:116-121."""
+    def side(w):
+        rho, p, u = w[0], w[1], w[2]
+        return rho, p, u, math.sqrt(gamma * p / rho), p / (gamma - 1) + 0.5 * rho * (w[2:] ** 2).sum()
+    rl, pl, ul, cl, el = side(wl)
+    rr, pr, ur, cr, er = side(wr)
+    sl, sr = math.sqrt(rl), math.sqrt(rr)
+    ur_ = (sl * wl[2:] + sr * wr[2:]) / (sl + sr)
+    h = ((el + pl) / sl + (er + pr) / sr) / (sl + sr)
+    a = math.sqrt(max((gamma - 1) * (h - 0.5 * (ur_ ** 2).sum()), 0.0))
+    al, ar = min(ur_[0] - a, ul - cl), max(ur_[0] + a, ur + cr)
+    ml, mr = rl * (ul - al), -rr * (ur - ar)
+    tl, tr = pl + (ul - al) * rl * ul, pr + (ur - ar) * rr * ur
+    return ml, mr, tl, tr
+
+
+def test_hllc_overflow_guard_branch():
+    """hllc.h:147-160: when |ml * tr| or |mr * tl| would exceed FLT_MAX / 10
+    the contact pressure is formed from the reciprocals 1/ml, 1/mr, which is
+    algebraically the same (ml tr + mr tl) / (ml + mr).  The flux is
+    homogeneous of degree one in (rho, p) at fixed velocity, so a Riemann
+    problem scaled by 1e20 (which takes this branch even in FP64) must give
+    1e20 times the unit-scale flux (which takes the ordinary branch)."""
+    f_max = 3.4028234663852886e38 / 10
+    lam = 1e20
+    for _ in range(100):
+        wl, wr = rand_hydro(), rand_hydro()
+        ml, mr, tl, tr = _hllc_gate(wl, wr, 1.4)
+        assert abs(ml * tr) < f_max and abs(mr * tl) < f_max
+        sl_, sr_ = wl.copy(), wr.copy()
+        sl_[:2] *= lam
+        sr_[:2] *= lam
+        ml, mr, tl, tr = _hllc_gate(sl_, sr_, 1.4)
+        assert abs(ml) > f_max / abs(tr) or abs(mr) > f_max / abs(tl)
+        f1 = oracle.hllc(wl, wr, 1.4)
+        fl = oracle.hllc(sl_, sr_, 1.4)
+        np.testing.assert_allclose(fl / lam, f1, rtol=1e-13, atol=1e-14)
+
+
+def test_hllc_degenerate_guard_branch():
+    """hllc.h:144-146: |ml + mr| < FLT_MIN * 10 sets the contact speed and
+    pressure to 0.  It needs c = 0 on both sides and no velocity jump,
+    i.e. p = 0: a cold gas at rest, whose exact flux is zero (and a cold
+    contact at rest: zero as well).  With the density and pressure floors
+    the pipeline never reaches it: ml + mr >= rho_floor sqrt(gamma
+    tmp_floor) ~ 4e-14.  For a cold gas in uniform motion the guard also
+    returns zero flux where f(U) = (rho u, ., rho u^2, ...) -- the
+    reference's behaviour as written, outside the floored states."""
+    for wl, wr in (([1.0, 0.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0, 0.0]),
+                   ([2.0, 0.0, 0.0, 0.0, 0.0], [0.5, 0.0, 0.0, 0.0, 0.0])):
+        wl, wr = np.array(wl), np.array(wr)
+        ml, mr, _, _ = _hllc_gate(wl, wr, 1.4)
+        assert abs(ml + mr) < 1.1754943508222875e-38 * 10
+        np.testing.assert_array_equal(oracle.hllc(wl, wr, 1.4), np.zeros(5))
+    cold = np.array([4.0, 0.0, 0.5, 0.0, 0.0])
+    np.testing.assert_array_equal(oracle.hllc(cold, cold, 1.4), np.zeros(5))
 
 
 # ------------------------------------------------------------------ HLLD

@@ -13,8 +13,11 @@ phys_functors.h) with what the mathematics fixes:
 
 The reference's reflect functor mirrors the tangential face field at -i
 (phys_functors.h:118-132) where the cell-centred values mirror at -i-1;
-the oracle follows that index as written.  No check above tells the two
-apart, so that one detail is parity unpinned.
+the oracle follows that index as written.  A mirror twin with a
+tangential field separates the two: the geometric -i-1 would equal the
+twin, -i does not, and the test asserts the difference (so -i-1 is
+excluded).  Nothing fixes -i itself, so that detail stays parity
+unpinned.
 """
 import math
 
@@ -72,10 +75,11 @@ def _cells(spec, blocks):
     return u
 
 
-def _slab_blocks(spec, mirror_from=None):
-    """1-D state along x (uniform in y, z), B = (0.6, 0, 0), v = (v_x, 0, 0):
-    the face field keeps no tangential component, so the flow is a
-    magnetised 1-D gas."""
+def _slab_blocks(spec, mirror_from=None, b_t=0.0):
+    """1-D state along x (uniform in y, z), B = (0.6, B_y(x), 0), v = (v_x,
+    0, 0).  With b_t = 0 the face field keeps no tangential component, so
+    the flow is a magnetised 1-D gas; b_t scales a tangential B_y(x), which
+    the mirror image negates (B is an axial vector)."""
     n = spec.n_cell_block
     tp = 2 * math.pi
     out = []
@@ -90,7 +94,9 @@ def _slab_blocks(spec, mirror_from=None):
         rho = np.broadcast_to(rho1[None, None, :], sh).copy()
         pre = np.broadcast_to(pre1[None, None, :], sh).copy()
         vel = [np.broadcast_to(vx1[None, None, :], sh).copy(), np.zeros(sh), np.zeros(sh)]
-        bfs = (np.full((n[2], n[1], n[0] + 1), 0.6), np.zeros((n[2], n[1] + 1, n[0])),
+        by1 = sgn * b_t * (0.5 + np.cos(tp * x))
+        bfs = (np.full((n[2], n[1], n[0] + 1), 0.6),
+               np.broadcast_to(by1[None, None, :], (n[2], n[1] + 1, n[0])).copy(),
                np.zeros((n[2] + 1, n[1], n[0])))
         out.append(problems._pack_block(n, rho, pre, vel, bfs, G))
     return out
@@ -115,6 +121,31 @@ def test_reflecting_wall_equals_mirrored_periodic_box():
     # drifts from an exact mirror of its left half
     np.testing.assert_allclose(uw[..., :5], ut[:, :, :32, :5], rtol=0, atol=1e-12)
     assert np.abs(uw[..., 2]).max() > 0.05
+
+
+def test_reflect_tangential_face_mirror_is_not_geometric():
+    """With a tangential B_y(x) and v_x != 0 at the wall, a wall built on the
+    geometric mirror of the tangential face field (ghost cell -1 <- cell 0,
+    index -i-1) would equal the mirrored periodic twin as the test above
+    does.  The reference's functor reads index -i (ghost -1 <- cell 1,
+    phys_functors.h:118-132), and so does the oracle: the two runs then
+    differ near the wall by far more than round-off.  This excludes -i-1;
+    no invariant fixes the -i choice itself."""
+    wall = RefinedMeshSpec((32, 4, 4), (32, 4, 4), x_max=(1.0, 1.0, 1.0),
+                           bnd=("ref", "ref", "per", "per", "per", "per"))
+    twin = RefinedMeshSpec((64, 4, 4), (64, 4, 4), x_max=(2.0, 1.0, 1.0))
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    ow = oracle.OracleMesh(wall, phys, n_threads=1)
+    ow.upload_blocks(_slab_blocks(wall, b_t=0.3))
+    ot = oracle.OracleMesh(twin, phys, n_threads=1)
+    ot.upload_blocks(_slab_blocks(twin, mirror_from=1.0, b_t=0.3))
+    assert ow.advance(10) == 10 and ot.advance(10) == 10
+    uw = np.asarray(ow.download_blocks()[0][0]).reshape(4, 4, 32, 8)
+    ut = np.asarray(ot.download_blocks()[0][0]).reshape(4, 4, 64, 8)
+    d = np.abs(uw[..., :5] - ut[:, :, :32, :5]).max(axis=(0, 1, 3))
+    assert d.max() > 1e-6, d
+    # the difference enters at the walls (with -i-1 the runs agree to 5e-16)
+    assert d[[0, 1, 30, 31]].min() > 10 * d[4:28].max(), d
 
 
 def test_reflecting_box_conserves_mass():
