@@ -2,7 +2,9 @@
 // and prob::cmd.  See mesh.h for what each piece restates.
 #include <cfloat>
 #include <cmath>
+#include <csignal>
 #include <cstdio>
+#include <cstring>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -52,7 +54,30 @@ void dual_t::copy_d2h() {
 }  // namespace block
 
 // ------------------------------------------------------------------ cycle
+namespace {
+// SIGINT: cycle::base_t::init installs the handler (cycle.cpp:32), as the
+// reference does.  The reference throws from inside its handler
+// (cycle.cpp:199-205); here the handler only records the signal, and the
+// evolve loop turns it into the same exception between kr_advance batches,
+// so the device work in flight finishes and the final snapshot sees a
+// consistent state.
+volatile std::sig_atomic_t g_signal = 0;
+extern "C" void on_signal(int signum) { g_signal = signum; }
+}  // namespace
+
+void cycle::base_t::sighdl(int signum) { on_signal(signum); }
+
+void cycle::base_t::poll_signal() {
+    const int s = g_signal;
+    if (s == 0) return;
+    g_signal = 0;
+    std::stringstream ss;
+    ss << "\nSignal : " << s << '\n';
+    throw std::runtime_error(ss.str());
+}
+
 void cycle::base_t::init(const input &args) {
+    std::signal(SIGINT, on_signal);
     args("cycle", "t_0", t);
     args("cycle", "t_lim", t_lim);
     args("cycle", "dt_init", dt_init);
@@ -212,22 +237,30 @@ void mesh_t::init(const input &args) {
 
 // Blocks of a same-level Cartesian mesh: order, geometry and the 26
 // neighbours with periodic wrap (meshgen.cpp:286-321, tree.cpp:77-84,
-// 177-197).  A thin axis (one cell) has no neighbours along it.
+// 177-197).  A thin axis (one cell) has no neighbours along it.  Block ids
+// follow the tree's lexicographic order (x slowest, geometry/region.h
+// :17-33), which is the reference's [mesh] dist_mode = 0 numbering
+// (meshgen.cpp:167-187): a restart of the reference from this build's
+// snapshot finds every block under the id it gives it.
 void mesh_t::build_blocks() {
     const auto &g = *p_mgn;
     bool periodic[3];
     for (int a = 0; a < 3; ++a) periodic[a] = g.bnd_kinds[2 * a].find("per") != std::string::npos;
     blocks_.clear();
+    tree_.clear();
+    set_tree_meta(periodic);
     bool refined = false;
     for (auto &r : g.ref_regions) refined |= r.level > 0;
     if (refined) return build_refined(periodic);
     const int nb = g.n_logic.prod();
+    auto lex_id = [&](const int *q) { return (q[0] * g.n_logic[1] + q[1]) * g.n_logic[2] + q[2]; };
     blocks_.resize(nb);
     for (int b = 0; b < nb; ++b) {
         block_t &k = blocks_[b];
         k.id = b;
-        k.reg.reg = type::idx_t(b % g.n_logic[0], (b / g.n_logic[0]) % g.n_logic[1],
-                                b / (g.n_logic[0] * g.n_logic[1]));
+        k.reg.reg = type::idx_t(b / (g.n_logic[1] * g.n_logic[2]), (b / g.n_logic[2]) % g.n_logic[1],
+                                b % g.n_logic[2]);
+        tree_[tree_key_t{0, k.reg.reg[0], k.reg.reg[1], k.reg.reg[2]}] = tree_node_t{b, false};
         k.geo.n_ceff = g.n_block;
         for (int a = 0; a < 3; ++a) {
             const int n = g.n_block[a];
@@ -254,9 +287,20 @@ void mesh_t::build_blocks() {
                         }
                         q[a] = v;
                     }
-                    k.nbr[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)] =
-                        ok ? (q[2] * g.n_logic[1] + q[1]) * g.n_logic[0] + q[0] : -1;
+                    k.nbr[((dz + 1) * 3 + (dy + 1)) * 3 + (dx + 1)] = ok ? lex_id(q) : -1;
                 }
+    }
+}
+
+// Tree-wide data of mesh/meshgen/meshgen.cpp:44-56: n_dim, the base-level
+// block counts (0 along an axis past n_dim, where the reference leaves
+// i_logic_lim unset) and the periodic flag per axis from [boundary] kinds.
+void mesh_t::set_tree_meta(const bool *periodic) {
+    const auto &g = *p_mgn;
+    tree_n_dim_ = g.n_dim;
+    for (int a = 0; a < 3; ++a) {
+        tree_reg_max_[a] = a < g.n_dim ? g.n_logic[a] : 0;
+        tree_periodic_[a] = periodic[a];
     }
 }
 
@@ -318,6 +362,7 @@ void mesh_t::build_refined(const bool *periodic) {
             for (int x = 0; x < g.n_logic[0]; ++x) tree.emplace(key_t{0, x, y, z}, false);
     int id = 0;
     for (auto &kv : tree) {
+        tree_[kv.first] = tree_node_t{kv.second ? -1 : id, kv.second};
         if (kv.second) continue;
         block_t k;
         k.id = id++;
@@ -339,13 +384,19 @@ void mesh_t::build_refined(const bool *periodic) {
     }
 }
 
-// A restart takes its blocks from the snapshot (the reference reads its
-// tree back from the file, meshgen.cpp:96-103): every block_<id>|level and
-// |i_logic pair, in id order, with the geometry of that level.
+// A restart takes its blocks from the snapshot's mesh_tree, as the
+// reference reads its tree back from the file (meshgen.cpp:96-104): the
+// unmasked nodes in the tree's order, numbered 0..n-1 as dist_default does
+// (meshgen.cpp:167-187), each reading its state from the tags of the id the
+// file gives it.  A snapshot without mesh_tree gives its blocks through the
+// per-block level and i_logic tags instead.
 void mesh_t::blocks_from_snapshot() {
-    const auto &g = *p_mgn;
     binary_io::base_t bio;
     bio.load(restart_file_);
+    if (bio.has_tag("mesh_tree")) {
+        const auto &b = bio.bytes("mesh_tree");
+        return blocks_from_tree(std::string(b.begin(), b.end()));
+    }
     std::map<int, block_t> found;
     for (const auto &t : bio.tags()) {
         const std::string suf = "|level";
@@ -358,26 +409,116 @@ void mesh_t::blocks_from_snapshot() {
         k.id = id;
         bio.read(&k.reg.level, 1, pre + "level");
         bio.read(k.reg.reg.x, 3, pre + "i_logic");
+        found[id] = k;
+    }
+    if (found.empty()) throw std::runtime_error("snapshot holds no blocks: " + restart_file_);
+    std::string bytes;
+    auto put = [&](const void *p, size_t n) { bytes.append((const char *)p, n); };
+    const size_t n = found.size();
+    put(&n, sizeof(n));
+    std::map<tree_key_t, int> leaves;
+    for (auto &kv : found) leaves[tree_key_t{kv.second.reg.level, kv.second.reg.reg[0],
+                                             kv.second.reg.reg[1], kv.second.reg.reg[2]}] = kv.first;
+    for (auto &kv : leaves) {
+        const int key[4] = {kv.first[1], kv.first[2], kv.first[3], kv.first[0]};
+        const int node[3] = {kv.second, kv.second, 0};
+        const char mask[4] = {0, 0, 0, 0};
+        put(key, sizeof(key));
+        put(node, sizeof(node));
+        put(mask, sizeof(mask));
+    }
+    blocks_from_tree(bytes);
+}
+
+// Parses the serialized tree (mesh/tree/tree.cpp:11-31 with
+// utilities/data_transfer/serialize.h: size_t node count, then per node
+// the region key {int x[3], int level} and the node {int id_l, id_g, rank;
+// bool mask} as raw structs, then n_dim, reg_max[3] and periodic[3]).
+// The trailing tree-wide fields are optional here (the tag-built tree above
+// has none); this build takes them from the parameters.
+void mesh_t::blocks_from_tree(const std::string &bytes) {
+    const auto &g = *p_mgn;
+    size_t off = 0;
+    auto get = [&](void *p, size_t n) {
+        if (off + n > bytes.size()) throw std::runtime_error("mesh_tree truncated");
+        std::memcpy(p, bytes.data() + off, n);
+        off += n;
+    };
+    size_t n = 0;
+    get(&n, sizeof(n));
+    std::map<tree_key_t, tree_node_t> nodes;
+    for (size_t i = 0; i < n; ++i) {
+        int key[4], node[3];
+        char mask[4];
+        get(key, sizeof(key));
+        get(node, sizeof(node));
+        get(mask, sizeof(mask));
+        nodes[tree_key_t{key[3], key[0], key[1], key[2]}] = tree_node_t{node[1], mask[0] != 0};
+    }
+    bool periodic[3];
+    for (int a = 0; a < 3; ++a) periodic[a] = g.bnd_kinds[2 * a].find("per") != std::string::npos;
+    bool refined = false;
+    for (auto &kv : nodes) refined |= kv.first[0] != 0 || kv.second.mask;
+    if (!refined) {
+        // same-level: the Cartesian blocks, each reading the tags of the id
+        // the file's tree gives its logical index
+        build_blocks();
+        for (auto &k : blocks_) {
+            auto it = nodes.find(tree_key_t{0, k.reg.reg[0], k.reg.reg[1], k.reg.reg[2]});
+            if (it == nodes.end()) throw std::runtime_error("mesh_tree lacks a block of the mesh");
+            k.file_id = it->second.id_g;
+        }
+        return;
+    }
+    blocks_.clear();
+    tree_.clear();
+    set_tree_meta(periodic);
+    int id = 0;
+    for (auto &kv : nodes) {
+        tree_[kv.first] = tree_node_t{kv.second.mask ? -1 : id, kv.second.mask};
+        if (kv.second.mask) continue;
+        block_t k;
+        k.id = id++;
+        k.file_id = kv.second.id_g;
+        k.reg.level = kv.first[0];
+        k.reg.reg = type::idx_t(kv.first[1], kv.first[2], kv.first[3]);
         k.geo.n_ceff = g.n_block;
         for (int a = 0; a < 3; ++a) {
-            const int n = g.n_block[a];
+            const int nc = g.n_block[a];
             auto loc = [&](type::float2_t i) -> type::float2_t {
-                if (n == 1) return i;
-                i = (k.reg.reg[a] + i / n) / (type::float2_t)(g.n_logic[a] << k.reg.level);
+                if (nc == 1) return i;
+                i = (k.reg.reg[a] + i / nc) / (type::float2_t)(g.n_logic[a] << k.reg.level);
                 return g.location(i, a);
             };
             k.geo.xf0[a] = (type::float_t)loc(0);
             k.geo.dx0[a] = (type::float_t)(loc(1) - k.geo.xf0[a]);
         }
         for (int j = 0; j < 27; ++j) k.nbr[j] = -1;
-        found[id] = k;
+        blocks_.push_back(k);
     }
-    if (found.empty()) throw std::runtime_error("snapshot holds no blocks: " + restart_file_);
-    bool refined = false;
-    for (auto &kv : found) refined |= kv.second.reg.level != 0;
-    if (!refined) return build_blocks();        // same-level: the Cartesian order and neighbours
-    blocks_.clear();
-    for (auto &kv : found) blocks_.push_back(kv.second);
+}
+
+// mesh_tree as the reference writes it (meshgen.cpp:459-466 ->
+// tree.cpp:11-20): the nodes in the tree's order with id_l = id_g (one
+// rank) and rank 0 for blocks, -1 for masked parents; padding bytes zero.
+void mesh_t::write_tree(binary_io::base_t &bio) const {
+    std::string bytes;
+    auto put = [&](const void *p, size_t n) { bytes.append((const char *)p, n); };
+    const size_t n = tree_.size();
+    put(&n, sizeof(n));
+    for (auto &kv : tree_) {
+        const int key[4] = {kv.first[1], kv.first[2], kv.first[3], kv.first[0]};
+        const int id = kv.second.id_g;
+        const int node[3] = {id, id, id < 0 ? -1 : 0};
+        const char mask[4] = {(char)kv.second.mask, 0, 0, 0};
+        put(key, sizeof(key));
+        put(node, sizeof(node));
+        put(mask, sizeof(mask));
+    }
+    put(&tree_n_dim_, sizeof(int));
+    put(tree_reg_max_, 3 * sizeof(int));
+    put(tree_periodic_, 3 * sizeof(bool));
+    bio.write(bytes.data(), bytes.size(), 1, "mesh_tree");
 }
 
 static int bnd_code(const std::string &k) {
@@ -461,7 +602,12 @@ void mesh_t::step() {
 // face field with its two ghost layers (zero here: ghosts are not stored).
 void mesh_t::save(const std::string &file_name) {
     binary_io::base_t bio;
-    bio.write_map(input_args_, "input_args");
+    // block ids follow the tree's order, the reference's dist_mode = 0: a
+    // restart of the reference from this file must number them that way
+    auto args = input_args_;
+    args["mesh|dist_mode"] = "0";
+    bio.write_map(args, "input_args");
+    write_tree(bio);
     double t, dtl, seed;
     check(kr_get_time(ctx_, &t, &dtl, &seed), ctx_, "kr_get_time");
     p_cyc->t = t;
@@ -515,7 +661,8 @@ void mesh_t::load_state() {
     for (size_t b = 0; b < blocks_.size(); ++b) {
         auto &d = *duals_[b];
         auto &h = d.h();
-        const std::string pre = "block_" + std::to_string(blocks_[b].id) + "|";
+        const int fid = blocks_[b].file_id >= 0 ? blocks_[b].file_id : blocks_[b].id;
+        const std::string pre = "block_" + std::to_string(fid) + "|";
         int nv = 0;
         bio.read(&nv, 1, pre + "hydro_cons_n_int");
         if (nv != h.hyd.n_var) throw std::runtime_error("snapshot model differs: " + pre);
@@ -550,6 +697,7 @@ void mesh_t::evolve() {
                 pull_time();
                 if (c.update_tout()) save(c.file_name());
             }
+            c.poll_signal();
             const int batch = outputs ? 1 : (int)std::min<long long>(left, c.n_display_freq);
             int done = 0;
             check(kr_advance(ctx_, batch, &done), ctx_, "kr_advance");

@@ -77,6 +77,7 @@ struct region_t {
 
 struct block_t {
     int id = 0;
+    int file_id = -1;      // id of the block's tags in the snapshot restarted from
     region_t reg;
     geo_t geo;
     int nbr[27];
@@ -160,6 +161,12 @@ struct base_t {
     std::string file_name();
     void save(binary_io::base_t &bio, double dt_seed);
     void load(const binary_io::base_t &bio, double &dt_seed);
+    // SIGINT (cycle.cpp:32, :199-205): the handler init installs records
+    // the signal; poll_signal, called by evolve between batches of cycles,
+    // throws the reference's "Signal : <n>" exception, which evolve turns
+    // into the final snapshot when final_output is set
+    static void sighdl(int signum);
+    void poll_signal();
 };
 }  // namespace cycle
 
@@ -286,10 +293,23 @@ public:
     void init_dynamics(int model, int precision, double gamma, double cfl, double rho_floor,
                        double tmp_floor);
 
+    // The block tree (mesh/tree/tree.h:28-67): every node by (level, x, y,
+    // z) in the tree's lexicographic order (geometry/region.h:17-33), with
+    // the node's global id (-1 for a masked, refined-away parent) and mask.
+    struct tree_node_t {
+        int id_g = -1;
+        bool mask = false;
+    };
+    typedef std::array<int, 4> tree_key_t;
+    const std::map<tree_key_t, tree_node_t> &tree() const { return tree_; }
+
 private:
     void build_blocks();
     void build_refined(const bool *periodic);
     void blocks_from_snapshot();
+    void blocks_from_tree(const std::string &bytes);
+    void set_tree_meta(const bool *periodic);
+    void write_tree(binary_io::base_t &bio) const;
     void pull_time();
     std::vector<std::function<void(const input &)>> reads_;
     std::map<int, std::function<void(const input &)>> inits_;
@@ -303,6 +323,10 @@ private:
     bool restart_ = false;
     std::string restart_file_;
     std::map<std::string, std::string> input_args_;
+    std::map<tree_key_t, tree_node_t> tree_;
+    int tree_n_dim_ = 0;
+    int tree_reg_max_[3] = {0, 0, 0};
+    bool tree_periodic_[3] = {false, false, false};
     double fix_w_[5] = {0, 0, 0, 0, 0}, fix_b_[3] = {0, 0, 0};   // "fixd" ghost state
 };
 

@@ -330,6 +330,7 @@ class RefinedMeshSpec:
             for y in range(self.nblk[1]):
                 for x in range(self.nblk[0]):
                     tree.setdefault((0, x, y, z), False)
+        self.tree = dict(tree)               # (level, x, y, z) -> masked
         self.leaves = [k for k in sorted(tree) if not tree[k]]
         self.n_levels = 1 + max(k[0] for k in self.leaves)
 

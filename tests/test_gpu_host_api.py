@@ -44,18 +44,29 @@ def run_pgen(name, par, tmp_path, extra=""):
     return np.fromfile(dump, dtype=np.float64), r.stdout
 
 
+def host_order(spec):
+    """MeshSpec block of each host-API block id: the host layer numbers the
+    blocks of a same-level mesh in the tree's order, x slowest (the
+    reference's dist_mode = 0), MeshSpec x fastest."""
+    nx, ny, nz = spec.nblk
+    return [(b % nz) * nx * ny + ((b // nz) % ny) * nx + b // (ny * nz)
+            for b in range(spec.n_blocks)]
+
+
 def split_dump(raw, spec, mhd):
     n = spec.n_cell_block
     nu = n[0] * n[1] * n[2] * (8 if mhd else 5)
     nf = bf_host_size(n) if mhd else 0
     per = nu + nf
     assert raw.size == 2 * spec.n_blocks * per
+    order = host_order(spec) if isinstance(spec, kratos_amd.MeshSpec) else list(
+        range(spec.n_blocks))
     states = []
     for s in range(2):
-        blocks = []
+        blocks = [None] * spec.n_blocks
         for b in range(spec.n_blocks):
             o = (s * spec.n_blocks + b) * per
-            blocks.append((raw[o:o + nu], raw[o + nu:o + per] if mhd else None))
+            blocks[order[b]] = (raw[o:o + nu], raw[o + nu:o + per] if mhd else None)
         states.append(spec.gather(blocks, mhd))
     return states
 
@@ -241,3 +252,54 @@ def test_blast_refined_snapshot_restart(gpu_lib, tmp_path):
     assert levels == {0, 1, 2}
     for k in blocks + [k.replace("hydro_cons", "field_bf") for k in blocks]:
         np.testing.assert_array_equal(np.frombuffer(a[k], "<f8"), np.frombuffer(c[k], "<f8"))
+
+
+def test_vortex_sigint_final_snapshot(gpu_lib, tmp_path):
+    """SIGINT during evolve (cycle.cpp:32, :169-181): the run stops between
+    cycles, writes its final snapshot (final_output = 1), and that snapshot
+    holds the state of an uninterrupted run of the same number of cycles,
+    bit for bit."""
+    import signal
+    import time as _time
+    exe = os.path.join(HOST, "bin", "vortex")
+    par = os.path.join(HOST, "usr_example", "vortex", "vortex.par")
+    over = tmp_path / "over_s.par"
+    over.write_text(f"[cycle]\nn_cycle_lim = 1000000\nn_display_freq = 1\nfinal_output = 1\n"
+                    f"prefix_output = {tmp_path}/S\n")
+    print("\n[case] host pgen vortex, SIGINT after 4 cycles", flush=True)
+    p = subprocess.Popen([exe, par, str(over)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True, cwd=tmp_path)
+    try:
+        seen = 0
+        t_end = _time.time() + 90
+        while seen < 4 and _time.time() < t_end:
+            line = p.stdout.readline()
+            if not line:
+                break
+            seen += line.startswith("cycle =")
+        assert seen == 4, "the run did not reach its fourth cycle"
+        p.send_signal(signal.SIGINT)
+        out, err = p.communicate(timeout=90)
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    print(out[-1500:], err[-1500:], flush=True)
+    assert p.returncode == 0, err
+    assert "Signal : 2" in err
+    s = _snap_fields(tmp_path / "S_00001.bin")
+    k = int(np.frombuffer(s["cycle"], "<i4")[0])          # cycles taken
+    assert k >= 4
+    # the reference's loop rule: n_cycle_lim = k - 1 takes k cycles
+    over2 = tmp_path / "over_r.par"
+    over2.write_text(f"[cycle]\nn_cycle_lim = {k - 1}\nfinal_output = 1\n"
+                     f"prefix_output = {tmp_path}/R\n")
+    r = subprocess.run(["timeout", "-k", "10", "120", exe, par, str(over2)], capture_output=True,
+                       text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    a = _snap_fields(tmp_path / "R_00001.bin")
+    assert a["time"] == s["time"]
+    blocks = sorted(x for x in a if x.endswith("|hydro_cons_field"))
+    assert len(blocks) == 8
+    for x in blocks + [x.replace("hydro_cons", "field_bf") for x in blocks]:
+        np.testing.assert_array_equal(np.frombuffer(a[x], "<f8"), np.frombuffer(s[x], "<f8"))
