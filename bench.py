@@ -152,6 +152,7 @@ def run_form(spec, phys, u, b, precision, steps, warmup, profile_cycles, path="a
     return ctx, {"wall": wall, "cycles": done, "gpu_ms": st["gpu_ms"], "prof": prof,
                  "warmup_cycles_run": warmup + extra,
                  "n_cells": st["n_cells"], "sweep_tile": st["sweep_tile"],
+                 "sweep_kernel": st["sweep_kernel"],
                  "tune_ms": [round(t, 4) for t in st["tune_ms"]]}
 
 
@@ -523,8 +524,12 @@ def main():
         bytes_launch = alg_bytes(dom, fs, fused) * units
         achieved = bytes_launch / (kms[dom] * 1e-3) / 1e9
         tl = r["sweep_tile"]
-        kname = (f"k_sweep<{'float' if head == 'mixed' else 'double'},{tl[0]},{tl[1]},{tl[2]},{tl[3]}>"
-                 if fused and dom == "update" else f"k_{dom}")
+        if fused and dom == "update" and r.get("sweep_kernel"):
+            kname = f"k_sweep_mx<{tl[0]},{tl[1]},{tl[2]}>"
+        elif fused and dom == "update":
+            kname = f"k_sweep<{'float' if head == 'mixed' else 'double'},{tl[0]},{tl[1]},{tl[2]},{tl[3]}>"
+        else:
+            kname = f"k_{dom}"
         cp = committed_profile(kname, workload)
         traffic = cp.get("bytes_per_launch")
         valu = cp.get("valu_wave_instr_per_launch")

@@ -104,13 +104,15 @@ typedef struct kr_stats {
     double alg_bytes_per_cell_update;  /* algorithmic HBM bytes            */
     int    sweep_tile[4];  /* fused kernel in use: tile x, y, threads, waves
                               per SIMD asked (zeros: staged path)           */
-    float  tune_ms[8];     /* autotuner: mean stage-kernel time per variant
+    float  tune_ms[16];    /* autotuner: mean stage-kernel time per variant
                               over its tuning cycle (0: not timed)          */
     int    tune_pending;   /* variants the autotuner has still to time: the
                               next that many cycles run eagerly, one
                               variant each                                  */
     int    graph_ready;    /* 1 once the cycle's hipGraph is instantiated
                               (kr_advance then only replays it)            */
+    int    sweep_kernel;   /* the fused MHD kernel in use: 0 k_sweep, 1 the
+                              mixed form's k_sweep_mx                       */
 } kr_stats;
 
 enum kr_kernel_kind {

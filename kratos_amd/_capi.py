@@ -61,9 +61,10 @@ class KrStats(C.Structure):
         ("kernel_calls", C.c_longlong * 8),
         ("alg_bytes_per_cell_update", C.c_double),
         ("sweep_tile", C.c_int * 4),
-        ("tune_ms", C.c_float * 8),
+        ("tune_ms", C.c_float * 16),
         ("tune_pending", C.c_int),
         ("graph_ready", C.c_int),
+        ("sweep_kernel", C.c_int),
     ]
 
 

@@ -5,6 +5,7 @@
 #include "kr_fused.h"
 #include "kr_fused_hydro.h"
 #include "kr_smr.h"
+#include "kr_sweep_mx.h"
 
 #include <hip/hip_runtime.h>
 
@@ -77,6 +78,8 @@ __global__ void __launch_bounds__(1024) k_max_reduce(const double *part, int npa
     }
 }
 
+
+
 }  // namespace kr
 
 using namespace kr;
@@ -99,16 +102,22 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 // graph replays between host checks of the stop rule in kr_advance
 constexpr int kAdvanceChunk = 64;
 
-// Fused stage-kernel variants: column tile, workgroup size, and the minimum
-// waves per SIMD asked of the compiler (1 = no constraint).  The autotuner
-// times every variant that fits the block shape (waves > 1 only for the
-// FP32 flux form) and keeps the fastest.
+// Fused stage-kernel variants: column tile, workgroup size, the minimum
+// waves per SIMD asked of the compiler (1 = no constraint), and the kernel:
+// k_sweep (mx = 0) or k_sweep_mx (mx = 1, the mixed form's kernel for MHD
+// meshes of one level without physical sides).  The autotuner times every
+// variant that fits the block shape and keeps the fastest; on a mesh that
+// k_sweep_mx serves it times only those (the two kernels round the FP32
+// fluxes differently, and the tuning cycles must leave the state any
+// single variant would), k_sweep's 4-wave builds being reachable there
+// only when kr_config.tile forces one.
 struct SweepVariant {
-    int tx, ty, nt, waves;
+    int tx, ty, nt, waves, mx;
 };
 constexpr SweepVariant kSweepVariants[] = {
-    {32, 8, 512, 1}, {32, 8, 512, 4}, {16, 16, 512, 1}, {16, 16, 512, 4},
-    {16, 8, 256, 1}, {8, 8, 192, 1}, {16, 8, 256, 4},
+    {32, 8, 512, 1, 0}, {32, 8, 512, 4, 0}, {16, 16, 512, 1, 0}, {16, 16, 512, 4, 0},
+    {16, 8, 256, 1, 0}, {8, 8, 192, 1, 0},  {16, 8, 256, 4, 0},  {16, 8, 256, 1, 1},
+    {32, 8, 512, 1, 1}, {16, 16, 512, 1, 1}, {8, 8, 192, 1, 1},
 };
 constexpr int kNumSweepVariants = sizeof(kSweepVariants) / sizeof(kSweepVariants[0]);
 
@@ -514,8 +523,6 @@ void commit(kr_ctx *c) {
                     if (G.lev[((dir[2] + 1) * 3 + (dir[1] + 1)) * 3 + (dir[0] + 1)] == 9)
                         G.phy |= 1 << (2 * a + sd);
                 }
-            if (c->mhd && G.phy)
-                throw std::invalid_argument("refined MHD meshes are periodic");
         }
     }
     // fused stage kernel: 3-D blocks divisible by a supported tile
@@ -523,14 +530,23 @@ void commit(kr_ctx *c) {
     // the first step.  On a refined mesh the fused kernel takes the blocks
     // with no cross-level direction; the others run, for MHD, its producer
     // form and the staged update, for hydro the staged kernels.  Refined
-    // hydro meshes with physical sides run the staged kernels throughout.
+    // meshes with physical sides (hydro and MHD) run the staged kernels
+    // throughout: they apply the side functors where cells are read, while
+    // the fused kernels read physical-side ghosts from halo blocks, which
+    // are built on one-level meshes only.
     bool phys_sides = false;
     for (int s = 0; s < 6; ++s) phys_sides |= c->cfg.bnd[s] != KR_PER;
-    if ((c->mhd || !c->smr || !phys_sides) && c->cfg.kernel_path != KR_PATH_STAGED) {
+    if (c->smr && phys_sides && c->cfg.kernel_path == KR_PATH_FUSED)
+        throw std::invalid_argument("the fused kernel does not run on refined meshes with "
+                                    "physical sides");
+    if (!(c->smr && phys_sides) && c->cfg.kernel_path != KR_PATH_STAGED) {
         const int tx = c->cfg.tile[0], ty = c->cfg.tile[1], tw = c->cfg.tile[2];
+        const bool mxmesh = c->mixed && c->mhd && !c->smr && !phys_sides;
         c->candidates.clear();
         for (int v = 0; v < kNumSweepVariants; ++v) {
             const SweepVariant &sv = kSweepVariants[v];
+            if (sv.mx && !mxmesh) continue;
+            if (mxmesh && !sv.mx && !(tx && tw > 1)) continue;
             if (tx && (sv.tx != tx || sv.ty != ty)) continue;
             if (tw && sv.waves != tw) continue;
             if (sv.waves > 1 && (!c->mixed || !c->mhd)) continue;
@@ -686,8 +702,23 @@ struct Launcher {
         case 3: sweep_t<FT, 16, 16, 512, f32 ? 4 : 1>(m, step, nb); break;
         case 4: sweep_t<FT, 16, 8, 256, 1>(m, step, nb); break;
         case 5: sweep_t<FT, 8, 8, 192, 1>(m, step, nb); break;
-        default: sweep_t<FT, 16, 8, 256, f32 ? 4 : 1>(m, step, nb); break;
+        case 6: sweep_t<FT, 16, 8, 256, f32 ? 4 : 1>(m, step, nb); break;
+        default:
+            if constexpr (f32) {
+                switch (variant) {
+                case 7: sweep_mx_t<16, 8, 256>(m, step, nb); break;
+                case 8: sweep_mx_t<32, 8, 512>(m, step, nb); break;
+                case 9: sweep_mx_t<16, 16, 512>(m, step, nb); break;
+                default: sweep_mx_t<8, 8, 192>(m, step, nb); break;
+                }
+            }
+            break;
         }
+    }
+    template <int TX, int TY, int NT>
+    void sweep_mx_t(const MeshView<float> &m, int step, unsigned nb) {
+        const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
+        hipLaunchKernelGGL((k_sweep_mx<TX, TY, NT>), dim3(nt, nb), dim3(NT), 0, c->stream, m, step);
     }
 
     template <class FT, bool MHD> void stage(int step) {
@@ -832,6 +863,7 @@ struct Launcher {
 
 void fill_tile_stats(kr_ctx *c) {
     for (int k = 0; k < 4; ++k) c->stats.sweep_tile[k] = 0;
+    c->stats.sweep_kernel = 0;
     c->stats.tune_pending = (c->fused && !c->tuned) ? (int)c->candidates.size() - c->tune_next : 0;
     c->stats.graph_ready = c->graph != nullptr;
     if (!c->fused) return;
@@ -840,7 +872,8 @@ void fill_tile_stats(kr_ctx *c) {
     c->stats.sweep_tile[1] = sv.ty;
     c->stats.sweep_tile[2] = sv.nt;
     c->stats.sweep_tile[3] = sv.waves;
-    for (int v = 0; v < kNumSweepVariants && v < 8; ++v) c->stats.tune_ms[v] = c->tune_ms[v];
+    c->stats.sweep_kernel = sv.mx;
+    for (int v = 0; v < kNumSweepVariants && v < 16; ++v) c->stats.tune_ms[v] = c->tune_ms[v];
 }
 
 // Tile autotuner on real cycles: while candidates are left, each cycle runs
@@ -1253,6 +1286,8 @@ int kr_divb_max(kr_ctx *c, double *divb) {
         return KR_OK;
     });
 }
+
+
 
 int kr_get_stats(kr_ctx *c, kr_stats *out) {
     if (!c || !out) return KR_EINVAL;

@@ -587,6 +587,7 @@ class Context:
             "kernel_calls": {k: s.kernel_calls[i] for i, k in enumerate(_capi.KERNEL_KINDS)},
             "alg_bytes_per_cell_update": s.alg_bytes_per_cell_update,
             "sweep_tile": tuple(s.sweep_tile),
+            "sweep_kernel": s.sweep_kernel,
             "tune_ms": tuple(s.tune_ms),
             "tune_pending": s.tune_pending,
             "graph_ready": bool(s.graph_ready),

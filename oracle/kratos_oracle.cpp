@@ -647,10 +647,6 @@ struct Mesh {
                 throw std::runtime_error("SMR blocks need an even size >= 4 per axis");
         nlev = 0;
         for (auto &b : blocks) nlev = std::max(nlev, b.level + 1);
-        if (cfg.mhd && nlev > 1)
-            for (int sd = 0; sd < 6; ++sd)
-                if (cfg.bnd[sd] != KRO_PER)
-                    throw std::runtime_error("refined MHD meshes are periodic");
         table.assign(nlev, std::vector<int>());
         for (int L = 0; L < nlev; ++L)
             table[L].assign((size_t)(nblk[0] << L) * (nblk[1] << L) * (nblk[2] << L), -1);
@@ -1563,6 +1559,12 @@ struct Mesh {
             if (!cfg.mhd) par_blocks([&](int bi) { phys_block(blocks[bi]); });
             else par_blocks([&](int bi) { phys_block_mhd(blocks[bi], step); });
             par_blocks([&](int bi) { smr_ghost(bi, 2); });
+            // MHD: the physical ghosts once more after the prolongation
+            // (mhd/boundary/keeper.h:288-293 runs the side kernels again
+            // after act_bnd), so that a ghost corner whose transverse source
+            // is a prolonged halo reads this substep's value; every other
+            // value is the same as in the first pass
+            if (cfg.mhd) par_blocks([&](int bi) { phys_block_mhd(blocks[bi], step); });
         }
         const double dt = dt_cur * (step == 0 ? 1.0 : 0.5);
         par_blocks([&](int bi) {

@@ -58,6 +58,7 @@ int main(void) {
     F(kr_stats, n_cells); F(kr_stats, kernel_ms); F(kr_stats, kernel_calls);
     F(kr_stats, alg_bytes_per_cell_update); F(kr_stats, sweep_tile); F(kr_stats, tune_ms);
     F(kr_block_desc, group); F(kr_stats, tune_pending); F(kr_stats, graph_ready);
+    F(kr_stats, sweep_kernel);
     return 0;
 }
 """

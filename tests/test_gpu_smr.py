@@ -21,6 +21,7 @@ G = 5.0 / 3.0
 TWO = [(1, (0.3, 0.3, 0.3), (0.6, 0.6, 0.6))]
 THREE = [(1, (0.25, 0.25, 0.25), (0.7, 0.7, 0.7)), (2, (0.45, 0.45, 0.45), (0.55, 0.55, 0.55))]
 BLAST3 = [(1, (-0.25,) * 3, (0.2,) * 3), (2, (-0.05,) * 3, (0.05,) * 3)]
+WALL = [(1, (0.0, 0.3, 0.3), (0.35, 0.6, 0.6))]        # level 1 reaching the x- side
 
 
 def _faces(spec, ba):
@@ -181,3 +182,35 @@ def test_unbalanced_mesh_rejected(gpu_lib):
     with pytest.raises(kratos_amd.KratosError, match="balanced|within one level"):
         ctx.upload_blocks(problems.smooth_blocks(spec, seed=1))
     ctx.close()
+
+
+@pytest.mark.parametrize("bnd", [("out", "fre", "ref", "ref", "fre", "out"), ("ref",) * 6])
+def test_smr_mhd_physical_sides_fp64(gpu_lib, bnd):
+    """Refined MHD meshes with physical sides (staged kernels: the side
+    functors applied where cells and face fields are read, the boundary-plane
+    EMF, the cross-level halo and corrections): FP64 bit-exact with the
+    oracle, div B at round-off."""
+    spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=WALL, bnd=bnd)
+    assert spec.n_levels == 2 and min(x for L, x, y, z in spec.leaves if L == 1) == 0
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    g, o, gt, ot, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=7), 6)
+    assert gt[0] == ot[0]
+    _assert_equal(spec, g, o, True)
+    assert divb <= 1e-12
+
+
+def test_smr_mhd_physical_sides_mixed(gpu_lib):
+    spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=WALL,
+                           bnd=("out", "fre", "ref", "ref", "fre", "out"))
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    g, o, _, _, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=7), 6, "mixed")
+    num = den = 0.0
+    for b in range(spec.n_blocks):
+        _, dx = spec.geometry(b)
+        v = dx[0] * dx[1] * dx[2]
+        gr = np.asarray(g[b][0]).reshape(-1, 8)[:, 0]
+        orr = np.asarray(o[b][0]).reshape(-1, 8)[:, 0]
+        num += np.abs(gr - orr).sum() * v
+        den += np.abs(orr).sum() * v
+    assert num / den <= 1e-5
+    assert divb <= 1e-12
