@@ -190,9 +190,27 @@ int kr_get_stats(kr_ctx *ctx, kr_stats *out);
 /* Block until all queued device work of the context has finished.       */
 int kr_sync(kr_ctx *ctx);
 
+/* Max |div B| of n_blocks blocks whose face fields the caller holds in
+ * device memory, in kr_upload's bf_aos layout (one slot with its ghost
+ * layers, block after block); dx_dev[3*b + a] (device memory) is block b's
+ * cell size along a.  The result is written to *divb_dev (device memory),
+ * in stream order on `stream` (a hipStream_t; NULL: the default stream),
+ * with the expression of kr_divb_max, so both give the same value.
+ * No counterpart in the reference: an allocation-free variant of
+ * kr_divb_max for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops).  It calls no hipMalloc / hipFree: its
+ * per-workgroup partial maxima live in a static device array of 65,536
+ * doubles (512 KiB per GPU, held until the process ends), so n_blocks is
+ * at most 65,536 (KR_EINVAL beyond, reason in kr_last_error(NULL)).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (two streams, two threads) race and must be ordered by the caller.    */
+int kr_divb_max_noalloc(const double *bf_dev, int n_blocks, const int n_cell_block[3],
+                        const double *dx_dev, double *divb_dev, void *stream);
+
 /* Message of the last failed call on ctx; with ctx == NULL, the reason the
- * last kr_create on this thread returned no context (the reference throws
- * std::runtime_error with such a text, device/hip/hip.cpp:8-16).         */
+ * last kr_create or kr_divb_max_noalloc on this thread failed (the
+ * reference throws std::runtime_error with such a text,
+ * device/hip/hip.cpp:8-16).                                              */
 const char *kr_last_error(kr_ctx *ctx);
 void kr_destroy(kr_ctx *ctx);
 

@@ -81,6 +81,8 @@ EXPORTS = {
     "kr_get_group_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kr_divb_max": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "kr_divb_max_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     "kr_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "kr_get_stats": (C.c_int, [C.c_void_p, C.POINTER(KrStats)]),
     "kr_sync": (C.c_int, [C.c_void_p]),

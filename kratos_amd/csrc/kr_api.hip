@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -79,6 +80,43 @@ __global__ void __launch_bounds__(1024) k_max_reduce(const double *part, int npa
 }
 
 
+// Partial maxima of |div B| over blocks whose face field is in the AoS host
+// layout of kr_upload (one slot, two ghost layers per side, a thin axis
+// with a single face), block after block in bf.  Same expression as
+// k_divb: s = sum over the active axes of (B_a(i + e_a) - B_a(i)) / dx_a.
+// Workgroup (x, b) covers cells x, x + wpb, ... of block b (grid stride).
+__global__ void __launch_bounds__(kThreads) k_divb_aos(const double *bf, int n0, int n1, int n2,
+                                                       const double *dx, int wpb, double *part) {
+    const int b = blockIdx.y;
+    const int n[3] = {n0, n1, n2};
+    int N[3], ndim = 0;
+    for (int a = 0; a < 3; ++a) {
+        N[a] = n[a] + (n[a] > 1) + 4;
+        if (n[a] > 1) ndim = a + 1;
+    }
+    const size_t ncell = (size_t)n0 * n1 * n2;
+    const double *B = bf + (size_t)b * N[0] * N[1] * N[2] * 3;
+    const size_t sh[3] = {3, (size_t)N[0] * 3, (size_t)N[0] * N[1] * 3};
+    double v = 0;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+         c += (size_t)wpb * blockDim.x) {
+        const int i = (int)(c % n0);
+        const int j = (int)((c / n0) % n1);
+        const int k = (int)(c / ((size_t)n0 * n1));
+        const size_t f0 = (((size_t)(k + 2) * N[1] + (j + 2)) * N[0] + (i + 2)) * 3;
+        double s = 0;
+        for (int a = 0; a < ndim; ++a) s += (B[f0 + sh[a] + a] - B[f0 + a]) / dx[3 * b + a];
+        v = pmax(v, kfabs(s));
+    }
+    const double r = block_min(-v);
+    if (threadIdx.x == 0) part[(size_t)b * wpb + blockIdx.x] = -r;
+}
+
+// Device scratch of kr_divb_max_noalloc: the per-workgroup partial maxima.
+// One copy per GPU, for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int kNoAllocParts = 65536;
+static __device__ __align__(256) double scratch[kNoAllocParts];
 
 }  // namespace kr
 
@@ -956,6 +994,22 @@ bool all_done(const std::vector<TimeState> &v) {
     return true;
 }
 
+// Address of the calling thread's current device's copy of the no-alloc
+// scratch, looked up once per device (with that device current) and kept.
+double *noalloc_scratch() {
+    static std::mutex mu;
+    static std::unordered_map<int, double *> addr;
+    int dev = 0;
+    KR_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    double *p = nullptr;
+    KR_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(kr::scratch)));
+    addr[dev] = p;
+    return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1288,6 +1342,41 @@ int kr_divb_max(kr_ctx *c, double *divb) {
 }
 
 
+int kr_divb_max_noalloc(const double *bf_dev, int n_blocks, const int n_cell_block[3],
+                        const double *dx_dev, double *divb_dev, void *stream) {
+    g_create_err.clear();
+    try {
+        if (!bf_dev || !n_cell_block || !dx_dev || !divb_dev || n_blocks < 1)
+            throw std::invalid_argument("kr_divb_max_noalloc: null pointer or no block");
+        for (int a = 0; a < 3; ++a)
+            if (n_cell_block[a] < 1)
+                throw std::invalid_argument("kr_divb_max_noalloc: cells per block < 1");
+        if (n_blocks > kr::kNoAllocParts)
+            throw std::invalid_argument("kr_divb_max_noalloc: " + std::to_string(n_blocks) +
+                                        " blocks exceed the scratch limit of " +
+                                        std::to_string(kr::kNoAllocParts));
+        double *part = noalloc_scratch();
+        const size_t ncell = (size_t)n_cell_block[0] * n_cell_block[1] * n_cell_block[2];
+        const int wpb = (int)std::max<size_t>(
+            1, std::min<size_t>(ceil_div(ncell, kr::kThreads), kr::kNoAllocParts / n_blocks));
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(kr::k_divb_aos, dim3(wpb, n_blocks), dim3(kr::kThreads), 0, st, bf_dev,
+                           n_cell_block[0], n_cell_block[1], n_cell_block[2], dx_dev, wpb, part);
+        hipLaunchKernelGGL(kr::k_max_reduce, dim3(1), dim3(1024), 0, st, (const double *)part,
+                           wpb * n_blocks, divb_dev);
+        KR_HIP(hipGetLastError());
+        return KR_OK;
+    } catch (const HipError &e) {
+        g_create_err = e.what();
+        return KR_EHIP;
+    } catch (const std::invalid_argument &e) {
+        g_create_err = e.what();
+        return KR_EINVAL;
+    } catch (const std::exception &e) {
+        g_create_err = e.what();
+        return KR_ESTATE;
+    }
+}
 
 int kr_get_stats(kr_ctx *c, kr_stats *out) {
     if (!c || !out) return KR_EINVAL;
