@@ -614,12 +614,27 @@ k_sweep(MeshView<FT> m, int step) {
         if (!PROD && L >= 1) {
             const int q = L - 1;                     // z face updated now
             const int fq = ring2(q), fpp = ring2(P);
-            auto set_face = [&](int c, int fx, int fy, int fz, double db) {
+            // Face-field update of one face (mhd/integrator/rk2.h:48-62): the
+            // two slots of every face a thread updates are read first
+            // (face_load), with the conserved state, and written last
+            // (face_store), so that the loads are in flight together.
+            struct FaceUpd {
+                double *bt;
+                double s, t;
+            };
+            auto face_load = [&](int c, int fx, int fy, int fz) {
                 const size_t f = fidx(m.n, x0 + fx, y0 + fy, fz);
-                double *Bs = m.BF + (((size_t)b * 2 + step) * 3 + c) * m.nface + f;
-                double *Bt = m.BF + (((size_t)b * 2 + (1 - step)) * 3 + c) * m.nface + f;
-                const double bs = *Bs;
-                *Bt = db + (step == 0 ? bs : (bs + *Bt) / 2);
+                FaceUpd u;
+                u.bt = m.BF + (((size_t)b * 2 + (1 - step)) * 3 + c) * m.nface + f;
+                u.s = m.BF[(((size_t)b * 2 + step) * 3 + c) * m.nface + f];
+                u.t = step == 0 ? 0.0 : *u.bt;
+                return u;
+            };
+            auto face_store = [&](const FaceUpd &u, double db) {
+                *u.bt = db + (step == 0 ? u.s : (u.s + u.t) / 2);
+            };
+            auto set_face = [&](int c, int fx, int fy, int fz, double db) {
+                face_store(face_load(c, fx, fy, fz), db);
             };
             // CT circulation of one face (mhd/integrator/integrator.h:429-457)
             auto ct = [&](double e1, double d1, double e2, double d2, double e3, double e4,
@@ -640,6 +655,9 @@ k_sweep(MeshView<FT> m, int step) {
                          dz = F32 ? sGeo[8] : g.dx[2];
             if (zint) {
                 const int x = cx_, y = cy_;
+                const FaceUpd uz = face_load(2, x, y, q);
+                const FaceUpd ux = face_load(0, x, y, P < 0 ? 0 : P);
+                const FaceUpd uy = face_load(1, x, y, P < 0 ? 0 : P);
                 // x + y partial of plane q (cells), finished next iteration
                 if (q < nz) {
                     // conserved state of plane q, loaded here so that it is not
@@ -687,18 +705,15 @@ k_sweep(MeshView<FT> m, int step) {
                     for (int v = 0; v < 5; ++v) part[v] = uu[v];
                 }
                 // z face q
-                set_face(2, x, y, q,
-                         ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
-                            sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
+                face_store(uz, ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
+                                  sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
                 if (P >= 0) {
                     // x face (x, y, P): a1 = y, a2 = z
-                    set_face(0, x, y, P,
-                             ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
-                                sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
+                    face_store(ux, ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
+                                      sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
                     // y face (x, y, P): a1 = z, a2 = x
-                    set_face(1, x, y, P,
-                             ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
-                                sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
+                    face_store(uy, ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
+                                      sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
                 }
             }
             if (P >= 0 && xedge) {

@@ -50,8 +50,10 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
     static_assert(M::WR <= NW, "one cell-pass item per thread at most");
     if (m.ts[m.geo[row_block(m)].grp].skip) return;
     const int tid = threadIdx.x, lane = tid & 63;
-    // wave index in a scalar register: the kind branches below are uniform
+    // the wave's role in a scalar register: the kind branches below are
+    // uniform; rtid is the thread's index within the roles
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rtid = wave * 64 + lane;
     const int b = row_block(m);
     const int ntx = m.n[0] / TX;
     const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
@@ -120,10 +122,10 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         }
     };
     // cell pass of plane L (see the header), one region cell per thread at
-    // most, on the waves from the last: its three normal face fields are
-    // requested first (cp_load) and written to LDS after the update, with
-    // the increments (cp_finish), so that the loads retire behind the
-    // update's own
+    // most, on the waves from the last: its three normal face fields (slot
+    // `step` of BF, which the launch only reads) are requested one
+    // iteration ahead (cp_load) and written to LDS with the increments
+    // (cp_finish), so that no wave waits for them
     // The item's coordinates are formed in the update phase from an opaque
     // copy of the thread index, so that they are not kept in registers
     // across the flux phase, where the kernel's register use peaks.
@@ -132,7 +134,7 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         bool corner;
     };
     auto cp_item = [&]() {
-        int t = tid;
+        int t = rtid;
         asm volatile("" : "+v"(t));
         CpItem it;
         it.r = (NW - 1 - (t >> 6)) * 64 + (t & 63);
@@ -189,10 +191,10 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         sBn[2][r] = (L >= 0 && L <= nz && !it.corner) ? (FT)bv[2] : (FT)0;
     };
 
-    // ---- per-thread persistent state (z pencil owners, tid < NZI)
+    // ---- per-thread persistent state (z pencil owners, rtid < NZI)
     int zx = 0, zy = 0;
-    const bool zown = tid < S::NZI;
-    if (zown) S::ring_item(tid, zx, zy);
+    const bool zown = rtid < S::NZI;
+    if (zown) S::ring_item(rtid, zx, zy);
     const bool zcorner = (zx < 0 || zx >= TX) && (zy < 0 || zy >= TY);
     const bool zint = zown && zx >= 0 && zx < TX && zy >= 0 && zy < TY;
     FT qL[8];
@@ -206,10 +208,11 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         part[v] = 0;
     }
     const int cx_ = zx, cy_ = zy;
-    const int et = tid - TX * TY;
+    const int et = rtid - TX * TY;
     const bool xedge = xlast && et >= 0 && et < TY;
     const bool yedge = ylast && et >= TY && et < TY + TX;
 
+    double bvn[3] = {0, 0, 0};       // normal face fields of the next cell pass
     issue_w(-2, ring3(-2));
     for (int L = -2; L <= nz + 1; ++L) {
         // ------------------------------------------------ 1. plane L arrives
@@ -396,20 +399,34 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         // plane L+1 into the slot of plane L-2, which phases 2 and 3 of this
         // iteration were the last to read
         const CpItem cpi = cp_item();
-        const bool cp_now = cpi.r < M::NR && L >= -1 && L <= nz;
-        double bv[3] = {0, 0, 0};
-        if (cp_now) cp_load(cpi, L, bv);
+        const bool cp_own = cpi.r < M::NR;
+        const bool cp_now = cp_own && L >= -1 && L <= nz;
+        const double bv[3] = {bvn[0], bvn[1], bvn[2]};
+        if (cp_own && L + 1 >= -1 && L + 1 <= nz) cp_load(cpi, L + 1, bvn);
         if (L + 1 <= nz + 1) issue_w(L + 1, ring3(L + 1));
 
         if (L >= 1) {
             const int q = L - 1;
             const int fq = ring2(q), fpp = ring2(P);
-            auto set_face = [&](int c, int fx, int fy, int fz, double db) {
+            // Face-field update of one face (mhd/integrator/rk2.h:48-62): its
+            // two slots are read first (face_load) for every face a thread
+            // updates, together with the conserved state, and written last
+            // (face_store), so that the loads of one iteration are in
+            // flight together rather than one face after the other.
+            struct FaceUpd {
+                double *bt;
+                double s, t;
+            };
+            auto face_load = [&](int c, int fx, int fy, int fz) {
                 const size_t f = fidx(m.n, x0 + fx, y0 + fy, fz);
-                double *Bs = m.BF + (((size_t)b * 2 + step) * 3 + c) * m.nface + f;
-                double *Bt = m.BF + (((size_t)b * 2 + (1 - step)) * 3 + c) * m.nface + f;
-                const double bs = *Bs;
-                *Bt = db + (step == 0 ? bs : (bs + *Bt) / 2);
+                FaceUpd u;
+                u.bt = m.BF + (((size_t)b * 2 + (1 - step)) * 3 + c) * m.nface + f;
+                u.s = m.BF[(((size_t)b * 2 + step) * 3 + c) * m.nface + f];
+                u.t = step == 0 ? 0.0 : *u.bt;
+                return u;
+            };
+            auto face_store = [&](const FaceUpd &u, double db) {
+                *u.bt = db + (step == 0 ? u.s : (u.s + u.t) / 2);
             };
             // CT circulation of one face (mhd/integrator/integrator.h:429-457)
             auto ct = [&](double e1, double d1, double e2, double d2, double e3, double e4, int a) {
@@ -423,6 +440,9 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
             const double dx = sGeo[6], dy = sGeo[7], dz = sGeo[8];
             if (zint) {
                 const int x = cx_, y = cy_;
+                const FaceUpd uz = face_load(2, x, y, q);
+                const FaceUpd ux = face_load(0, x, y, P < 0 ? 0 : P);
+                const FaceUpd uy = face_load(1, x, y, P < 0 ? 0 : P);
                 if (q < nz) {
                     const size_t c = cidx(m.n, x0 + x, y0 + y, q);
                     const double *Us = m.U + ((size_t)b * 2 + step) * 5 * nc + c;
@@ -453,29 +473,26 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) part[v] = uu[v];
                 }
-                set_face(2, x, y, q,
-                         ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
-                            sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
+                face_store(uz, ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
+                                  sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
                 if (P >= 0) {
-                    set_face(0, x, y, P,
-                             ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
-                                sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
-                    set_face(1, x, y, P,
-                             ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
-                                sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
+                    face_store(ux, ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
+                                      sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
+                    face_store(uy, ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
+                                      sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
                 }
             }
             if (P >= 0 && xedge) {
                 const int x = TX, y = et;
-                set_face(0, x, y, P,
-                         ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
-                            sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
+                const FaceUpd ux = face_load(0, x, y, P);
+                face_store(ux, ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
+                                  sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
             }
             if (P >= 0 && yedge) {
                 const int x = et - TY, y = TY;
-                set_face(1, x, y, P,
-                         ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
-                            sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
+                const FaceUpd uy = face_load(1, x, y, P);
+                face_store(uy, ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
+                                  sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
             }
         }
         if (cp_now) cp_finish(cpi, L, bv);
