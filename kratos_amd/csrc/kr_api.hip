@@ -198,10 +198,12 @@ struct kr_ctx {
     int nfill[2] = {0, 0};
     int2 *d_ftask = nullptr;        // faces whose neighbour is finer
     int nftask = 0;
-    int *d_xblk = nullptr, *d_rblk = nullptr;   // blocks with cross-level / refined sides
-    int nxblk = 0, nrblk = 0;
+    int *d_xblk = nullptr;          // blocks with cross-level sides
+    int nxblk = 0;
     int *d_pure = nullptr;          // blocks with no cross-level direction (fused kernel)
     int npure = 0;
+    int *d_sweep = nullptr;         // the pure blocks, then the cross-level ones: the rows
+                                    // of the one stage launch of the fused kernel (MHD)
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -213,7 +215,6 @@ struct kr_ctx {
         MeshView<FT> m{};
         for (int a = 0; a < 3; ++a) m.n[a] = n[a];
         m.ndim = ndim;
-        m.nblocks = (int)descs.size();
         m.nv = nv;
         for (int s = 0; s < 6; ++s) m.bnd[s] = cfg.bnd[s];
         m.gamma = cfg.gamma;
@@ -245,15 +246,15 @@ void release(kr_ctx *c) {
     c->graph = nullptr;
     void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE, c->EC,
                     c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
-                    c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk, c->d_rblk,
-                    c->d_pure, c->d_pfill};
+                    c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk,
+                    c->d_pure, c->d_pfill, c->d_sweep};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->d_table = nullptr;
     c->d_fill[0] = c->d_fill[1] = nullptr;
     c->d_pfill = nullptr;
     c->d_ftask = nullptr;
-    c->d_xblk = c->d_rblk = c->d_pure = nullptr;
+    c->d_xblk = c->d_pure = c->d_sweep = nullptr;
     c->U = c->BF = nullptr;
     c->W = c->EE = c->EC = nullptr;
     c->F[0] = c->F[1] = c->F[2] = nullptr;
@@ -341,7 +342,7 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     }
     std::vector<int4> fill[2];
     std::vector<int2> ftask;
-    std::vector<int> xblk, rblk, pure;
+    std::vector<int> xblk, pure;
     int nhalo = 0;
     for (size_t b = 0; b < nb; ++b) {
         const kr_block_desc &d = c->descs[b];
@@ -387,15 +388,25 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
                 }
                 if (lv == 1 || lv == -1) G.xface |= 1 << (2 * a + sd);
             }
-        bool any = false, fin = false;
-        for (int k = 0; k < 27; ++k) {
-            any |= G.lev[k] == 1 || G.lev[k] == -1;
-            fin |= G.lev[k] == 1;
-        }
+        // the four cells around an edge of component c in regions (r1, r2):
+        // offsets -1 / 0 along c1 and c2, outside the block on a surface side
+        for (int c = 0; c < 3; ++c)
+            for (int r1 = 0; r1 < 3; ++r1)
+                for (int r2 = 0; r2 < 3; ++r2) {
+                    bool coarser = false;
+                    for (int o1 = -1; o1 <= 0; ++o1)
+                        for (int o2 = -1; o2 <= 0; ++o2) {
+                            int d[3] = {0, 0, 0};
+                            d[(c + 1) % 3] = (r1 == 0 && o1 < 0) ? -1 : (r1 == 2 && o1 == 0 ? 1 : 0);
+                            d[(c + 2) % 3] = (r2 == 0 && o2 < 0) ? -1 : (r2 == 2 && o2 == 0 ? 1 : 0);
+                            coarser |= G.lev[((d[2] + 1) * 3 + (d[1] + 1)) * 3 + (d[0] + 1)] == -1;
+                        }
+                    if (coarser) G.wbmask |= 1 << ((c * 3 + r1) * 3 + r2);
+                }
+        bool any = false;
+        for (int k = 0; k < 27; ++k) any |= G.lev[k] == 1 || G.lev[k] == -1;
         if (any) xblk.push_back((int)b);
         else pure.push_back((int)b);
-        if (G.rface) rblk.push_back((int)b);
-        (void)fin;
     }
     // a prolongation's slopes read the coarser block's cells one step past
     // the covering cell; those must never be prolonged themselves
@@ -465,10 +476,11 @@ void build_smr(kr_ctx *c, std::vector<BlockGeo> &g) {
     c->d_pure = dalloc<int>(pure.size());
     if (!pure.empty())
         KR_HIP(hipMemcpy(c->d_pure, pure.data(), pure.size() * sizeof(int), hipMemcpyHostToDevice));
-    c->nrblk = (int)rblk.size();
-    c->d_rblk = dalloc<int>(rblk.size());
-    if (!rblk.empty())
-        KR_HIP(hipMemcpy(c->d_rblk, rblk.data(), rblk.size() * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> rows(pure);
+    rows.insert(rows.end(), xblk.begin(), xblk.end());
+    c->d_sweep = dalloc<int>(rows.size());
+    if (!rows.empty())
+        KR_HIP(hipMemcpy(c->d_sweep, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice));
 }
 
 // Lay out the pools once every block is known (the reference allocates per
@@ -667,6 +679,16 @@ void commit(kr_ctx *c) {
         if (c->n[0] % 16 == 0 && c->n[1] % 8 == 0) c->prod = 0;
         else if (c->n[0] % 8 == 0 && c->n[1] % 8 == 0) c->prod = 1;
     }
+    // mixed form: one stage launch runs both forms (k_sweep<..., PROD = 2>),
+    // so the pure blocks take the producer's tile; no tuning.  The FP64 form
+    // keeps two launches (the combined kernel would run at one wave per SIMD)
+    if (c->prod >= 0 && c->mixed) {
+        const int want = c->prod == 0 ? 4 : 5;      // 16x8/256, 8x8/192
+        c->candidates.assign(1, want);
+        c->variant = want;
+        c->tuned = true;
+        c->tune_next = 0;
+    }
     // cell-centred E for the staged EMF kernel, when it runs
     if (c->mhd && (!c->fused || (c->smr && c->prod < 0)))
         KR_HIP(hipMalloc(&c->EC, nb * 3 * c->ncell * fs));
@@ -698,7 +720,7 @@ struct Launcher {
         evs.push_back({kind, {a, b}});
     }
 
-    template <class FT, int TX, int TY, int NT, int W, bool PROD = false>
+    template <class FT, int TX, int TY, int NT, int W, int PROD = 0>
     void sweep_t(const MeshView<FT> &m, int step, unsigned nb) {
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
         if (c->npfill)                      // physical sides (one-level meshes only)
@@ -710,8 +732,13 @@ struct Launcher {
     }
     // fluxes and edge EMFs of the listed blocks into F / EE (no update)
     template <class FT> void sweep_prod(const MeshView<FT> &m, int step, unsigned nb) {
-        if (c->prod == 0) sweep_t<FT, 16, 8, 256, 1, true>(m, step, nb);
-        else sweep_t<FT, 8, 8, 192, 1, true>(m, step, nb);
+        if (c->prod == 0) sweep_t<FT, 16, 8, 256, 1, 1>(m, step, nb);
+        else sweep_t<FT, 8, 8, 192, 1, 1>(m, step, nb);
+    }
+    template <class FT> void sweep_both(const MeshView<FT> &m, int step, unsigned nb) {
+        static_assert(std::is_same_v<FT, float>, "the combined launch is the mixed form's");
+        if (c->prod == 0) sweep_t<FT, 16, 8, 256, 1, 2>(m, step, nb);
+        else sweep_t<FT, 8, 8, 192, 1, 2>(m, step, nb);
     }
 
     template <class FT, int TX, int TY, int NT>
@@ -809,7 +836,19 @@ struct Launcher {
         MeshView<FT> ms = m;
         unsigned ns = nb;
         if (c->smr && c->fused) {
-            if (c->npure) {
+            bool both = false;
+            if constexpr (MHD && std::is_same_v<FT, float>)
+                if (c->prod >= 0) {
+                    // one launch: the full form on the pure blocks' rows, the
+                    // producer form (fluxes and EMFs into F / EE) on the others
+                    MeshView<FT> mb = m;
+                    mb.blist = c->d_sweep;
+                    mb.nfull = c->npure;
+                    run(KR_K_UPDATE,
+                        [&] { sweep_both<FT>(mb, step, (unsigned)(c->npure + c->nxblk)); }, true);
+                    both = true;
+                }
+            if (!both && c->npure) {
                 MeshView<FT> mp = m;
                 mp.blist = c->d_pure;
                 if constexpr (MHD)
@@ -823,7 +862,10 @@ struct Launcher {
         }
         if (ns == 0) return;
         if (MHD && c->smr && c->prod >= 0) {
-            run(KR_K_FLUX_X, [&] { sweep_prod<FT>(ms, step, ns); });
+            // producer form: fluxes and EMFs into F / EE (the mixed form ran
+            // it in the launch above)
+            if constexpr (!std::is_same_v<FT, float>)
+                run(KR_K_FLUX_X, [&] { sweep_prod<FT>(ms, step, ns); });
         } else {
         if (c->ndim > 2)
             run(KR_K_FLUX_Z, [&] {
@@ -845,37 +887,40 @@ struct Launcher {
                                    c->stream, ms);
             });
         }
-        if (c->smr) {                       // act_flx: EMF and flux restriction
-            if (MHD && c->nxblk) {
-                run(KR_K_EMF, [&] {
-                    hipLaunchKernelGGL(k_smr_emf_coarse<FT>, dim3(ceil_div(c->nface, kSmrThreads), c->nxblk),
-                                       dim3(kSmrThreads), 0, c->stream, m, c->sv, (const int *)c->d_xblk);
-                });
-                run(KR_K_EMF, [&] {
-                    hipLaunchKernelGGL(k_smr_emf_wb<FT>, dim3(ceil_div(c->nface, kSmrThreads), c->nxblk),
-                                       dim3(kSmrThreads), 0, c->stream, m, (const int *)c->d_xblk);
+        if (c->smr) {
+            // act_flx in one launch (EMF restriction on the coarser blocks'
+            // shared edges, flux restriction on the faces whose neighbour is
+            // finer), then the update with the finer side's EMF write-back and
+            // intg_flx folded in
+            const int nx = MHD ? c->nxblk : 0;
+            if (nx + c->nftask) {
+                // largest face side; one row covers an edge box (MHD) or at
+                // least one face side, tpr face tasks per row
+                const int fmax = c->n[0] * c->n[1] * c->n[2] /
+                                 std::min(c->n[0], std::min(c->n[1], c->n[2]));
+                const size_t w = ceil_div(std::max(MHD ? c->nface : (size_t)0, (size_t)fmax),
+                                          kSmrThreads) * kSmrThreads;
+                const int tpr = (int)(w / fmax);
+                const int frows = (int)ceil_div((size_t)c->nftask, (size_t)tpr);
+                run(MHD ? KR_K_EMF : KR_K_UPDATE, [&] {
+                    hipLaunchKernelGGL((k_smr_restrict<FT, NV>), dim3(w / kSmrThreads, nx + frows),
+                                       dim3(kSmrThreads), 0, c->stream, m, c->sv,
+                                       (const int *)c->d_xblk, nx, (const int2 *)c->d_ftask,
+                                       c->nftask, fmax, tpr);
                 });
             }
-            if (c->nftask)
-                run(KR_K_UPDATE, [&] {
-                    hipLaunchKernelGGL((k_smr_flux<FT, NV>),
-                                       dim3(ceil_div((size_t)c->n[0] * c->n[1] * c->n[2] /
-                                                         std::min(c->n[0], std::min(c->n[1], c->n[2])),
-                                                     kSmrThreads),
-                                            c->nftask),
-                                       dim3(kSmrThreads), 0, c->stream, m, c->sv, (const int2 *)c->d_ftask);
-                });
+            run(KR_K_UPDATE, [&] {
+                hipLaunchKernelGGL((k_smr_update<FT, MHD>),
+                                   dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), ns), blk, 0,
+                                   c->stream, ms, step);
+            });
+            return;
         }
         run(KR_K_UPDATE, [&] {
             hipLaunchKernelGGL((k_update<FT, MHD>),
                                dim3(ceil_div(MHD ? c->nface : c->ncell, kThreads), ns), blk, 0,
                                c->stream, ms, step);
         });
-        if (c->smr && c->nrblk)             // intg_flx: the refined faces
-            run(KR_K_UPDATE, [&] {
-                hipLaunchKernelGGL(k_smr_intg<FT>, dim3(ceil_div(c->ncell, kSmrThreads), c->nrblk),
-                                   dim3(kSmrThreads), 0, c->stream, m, (const int *)c->d_rblk, step);
-            });
     }
 
     template <class FT, bool MHD> void cycle_t() {

@@ -160,7 +160,11 @@ __device__ __forceinline__ double edge_emf<double>(double ec11, double ec10, dou
 // EMFs) can act on them before k_update.  Where a transverse ghost line
 // runs next to a face shared with another level, the normal field is the
 // cell value of W there (mhd/integrator/data.h:108-140), as in load_bf.
-template <class FT, int TX, int TY, int NT, int WAVES, bool PROD = false, bool PHY = false>
+//
+// PROD = 2 (refined meshes): one launch for both kinds of block, the grid
+// rows below m.nfull updating (the full form), the rows from m.nfull on
+// producing (the producer form); a workgroup takes its row's form.
+template <class FT, int TX, int TY, int NT, int WAVES, int PROD = 0, bool PHY = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_sweep(MeshView<FT> m, int step) {
     using S = TileShape<TX, TY>;
@@ -170,6 +174,7 @@ k_sweep(MeshView<FT> m, int step) {
     if (m.ts[m.geo[row_block(m)].grp].skip) return;
     const int tid = threadIdx.x;
     const int b = row_block(m);
+    const bool prod = PROD == 1 || (PROD == 2 && (int)blockIdx.y >= m.nfull);
     const int ntx = m.n[0] / TX;
     const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
     const int nz = m.n[2];
@@ -275,7 +280,7 @@ k_sweep(MeshView<FT> m, int step) {
                                         fidx(m.n, p[0], p[1], p[2])]);
             }
         }
-        if constexpr (PROD) {
+        if (prod) {
             if (out && g.xface) {
                 int bit = 0;
 #pragma unroll
@@ -402,7 +407,7 @@ k_sweep(MeshView<FT> m, int step) {
                     sFz[0][gz] = flx[RHO];
                     sFz[1][gz] = flx[BCC + 2];
                     sFz[2][gz] = -flx[BCC + 1];
-                    if (PROD && zint) {
+                    if (prod && zint) {
                         // the block's z face (x, y, p): hydro flux for the staged update
                         const int q3[3] = {x0 + cx_, y0 + cy_, p};
                         FT *F = m.F[2] + (size_t)b * 8 * m.nfa[2] + xidx(m.n, 2, q3);
@@ -487,7 +492,7 @@ k_sweep(MeshView<FT> m, int step) {
                 sFh[0][it] = flx[ENE];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) sFh[1 + (a + k) % 3][it] = flx[MOM + k];
-                if constexpr (PROD) {
+                if (prod) {
                     // a face of the block (not a transverse ghost line, not a
                     // ghost plane): hydro flux for the staged update
                     if (!ghost_plane && (xf ? (fy >= 0 && fy < TY) : (fx >= 0 && fx < TX))) {
@@ -571,7 +576,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFe[fe][0][S::fxi(x, y)], sFe[fe][0][S::fxi(x, y - 1)]);
                     if (PHY && g.phy) e = bplane(2, x, y, e);
                     sEz[fe][S::ezi(x, y)] = e;
-                    if constexpr (PROD)
+                    if (prod)
                         m.EE[((size_t)b * 3 + 2) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 } else if (it < S::NEZ + S::NEX) {
                     const int k = it - S::NEZ;
@@ -585,7 +590,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFe[fe][0][S::fyi(x, y)], sFe[fm][0][S::fyi(x, y)]);
                     if (PHY && g.phy) e = bplane(0, x, y, e);
                     sEx[fe][S::exi(x, y)] = e;
-                    if constexpr (PROD)
+                    if (prod)
                         m.EE[((size_t)b * 3 + 0) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 } else {
                     const int k = it - S::NEZ - S::NEX;
@@ -599,7 +604,7 @@ k_sweep(MeshView<FT> m, int step) {
                         sFz[0][S::zg(x, y)], sFz[0][S::zg(x - 1, y)]);
                     if (PHY && g.phy) e = bplane(1, x, y, e);
                     sEy[fe][S::eyi(x, y)] = e;
-                    if constexpr (PROD)
+                    if (prod)
                         m.EE[((size_t)b * 3 + 1) * m.nface + fidx(m.n, x0 + x, y0 + y, p)] = e;
                 }
             }
@@ -611,7 +616,7 @@ k_sweep(MeshView<FT> m, int step) {
         if (L + 1 <= nz + 1) issue_w(L + 1, ring3(L + 1));
 
         // ------------------------------------------------ 4. update
-        if (!PROD && L >= 1) {
+        if (!prod && L >= 1) {
             const int q = L - 1;                     // z face updated now
             const int fq = ring2(q), fpp = ring2(P);
             // Face-field update of one face (mhd/integrator/rk2.h:48-62): the

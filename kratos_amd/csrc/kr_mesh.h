@@ -29,6 +29,9 @@ struct BlockGeo {
     int reg[3];        // logical index of the block at its level
     int rface;         // bit (2*axis + side): the face neighbour is finer (refine_flag)
     int xface;         // bit (2*axis + side): the face neighbour is at another level
+    int wbmask;        // bit (3*c + r1)*3 + r2: an edge of component c whose two other
+                       // coordinates lie in regions r1, r2 (0 low surface, 1 inside, 2 high
+                       // surface) touches a coarser leaf (the finer side's write-back)
     signed char lev[27];  // level of the leaf covering each direction, relative
                           // (-1 coarser, 0 same, +1 finer; 9 outside the domain)
 };
@@ -76,7 +79,8 @@ template <class FT>
 struct MeshView {
     int n[3];          // cells per block along each axis
     int ndim;          // axes with more than one cell
-    int nblocks;
+    int nfull;         // k_sweep<..., PROD = 2>: grid rows below run the full form, the
+                       // rest the producer form
     int nv;            // variables in W and F: 5 hydro, 8 MHD
     int bnd[6];        // boundary kind per side (0 per, 1 out, 2 fre, 3 ref, 4 fixd)
     double gamma, cfl, d_floor, t_floor;

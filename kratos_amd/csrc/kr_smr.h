@@ -129,16 +129,14 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_fill(MeshView<FT> m, SmrVie
 }
 
 // Flux restriction onto a face whose neighbour is finer.  Task {block,
-// 2*axis + side}; one thread per coarse face.
+// 2*axis + side}; thread t takes coarse face t of the task's side.
 template <class FT, int NV>
-__global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrView s,
-                                                         const int2 *task) {
-    const int2 tk = task[blockIdx.y];
+__device__ __forceinline__ void smr_flux_at(const MeshView<FT> &m, const SmrView &s, int2 tk,
+                                            int t) {
     const int b = tk.x, a = tk.y >> 1, sd = tk.y & 1;
     const BlockGeo &g = m.geo[b];
     if (m.ts[g.grp].skip) return;
     const int a1 = mod3(a + 1), a2 = mod3(a + 2);
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.n[a1] * m.n[a2]) return;
     const int j1 = t % m.n[a1], j2 = t / m.n[a1];
     int G[3];
@@ -185,36 +183,6 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_flux(MeshView<FT> m, SmrVie
     for (int v = 0; v < NH; ++v) Fc[(size_t)v * m.nfa[a]] = fb[v] / norm;
 }
 
-// intg_ker: add the corrected flux of every refined face to the adjacent
-// cell of the target slot, z sides first, then y, then x.  One thread per
-// cell of the listed blocks (interior cells return at once).
-template <class FT>
-__global__ void __launch_bounds__(kSmrThreads) k_smr_intg(MeshView<FT> m, const int *blocks,
-                                                         int step) {
-    const int b = blocks[blockIdx.y];
-    const BlockGeo &g = m.geo[b];
-    if (m.ts[g.grp].skip) return;
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.ncell) return;
-    const int i[3] = {(int)(c % m.n[0]), (int)((c / m.n[0]) % m.n[1]),
-                      (int)(c / ((size_t)m.n[0] * m.n[1]))};
-    const double dt = m.ts[g.grp].dt_use * (step == 0 ? 1.0 : 0.5);
-    double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * m.ncell + c;
-    for (int a = 2; a >= 0; --a)
-        for (int sd = 0; sd <= 1; ++sd) {
-            if (!((g.rface >> (2 * a + sd)) & 1)) continue;
-            if (i[a] != (sd ? m.n[a] - 1 : 0)) continue;
-            int q[3] = {i[0], i[1], i[2]};
-            q[a] += sd;
-            const FT *F = m.F[a] + (size_t)b * m.nv * m.nfa[a] + xidx(m.n, a, q);
-            const FT ds = (FT)g.sfc[a];
-            const FT dv = (FT)g.vol * (FT)(sd ? -1 : 1);
-            const double conv = dt * (double)ds / (double)dv;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Ut[(size_t)v * m.ncell] += conv * (double)F[(size_t)v * m.nfa[a]];
-        }
-}
-
 // Levels of the four cells around edge (component a) at local position i:
 // bit 0 set if a finer leaf shares it (first such cell in fo1/fo2), bit 1 if
 // a coarser one does.
@@ -253,15 +221,14 @@ __device__ __forceinline__ bool on_block_surface(const MeshView<FT> &m, const in
 
 // Coarser side: edges shared with a finer leaf take the length-weighted
 // mean of the two finer edges (mhd/boundary/comm_functions.h:795-864 and
-// :919-954, the packed value).  Reads only finer blocks' edges, writes only
-// this block's: one launch is race-free.
+// :919-954, the packed value).  Reads only finer blocks' edges, which
+// nothing writes in the stage (the finer side's write-back is applied where
+// the update reads them, k_smr_update), and writes only this block's.
 template <class FT>
-__global__ void __launch_bounds__(kSmrThreads) k_smr_emf_coarse(MeshView<FT> m, SmrView s,
-                                                               const int *blocks) {
-    const int b = blocks[blockIdx.y];
+__device__ __forceinline__ void smr_emf_coarse_at(const MeshView<FT> &m, const SmrView &s, int b,
+                                                  size_t t) {
     const BlockGeo &g = m.geo[b];
     if (m.ts[g.grp].skip) return;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.nface) return;
     const int i[3] = {(int)(t % (m.n[0] + 1)), (int)((t / (m.n[0] + 1)) % (m.n[1] + 1)),
                       (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)))};
@@ -297,32 +264,88 @@ __global__ void __launch_bounds__(kSmrThreads) k_smr_emf_coarse(MeshView<FT> m, 
     }
 }
 
-// Finer side: edge pairs on a coarser edge line shared with a coarser leaf
-// take their length-weighted mean (the write-back of comm_functions.h
-// :855-861, 950-952).  Runs after k_smr_emf_coarse read the originals.
+// Finer side: an edge pair on a coarser edge line shared with a coarser
+// leaf takes its length-weighted mean (the write-back of comm_functions.h
+// :855-861, 950-952).  Evaluated where the update reads the edge: the value
+// of edge (component c, edge-box index idx, coordinates j) of block b.
+// Pairs start at even indices along c and lie on even transverse lines;
+// whether the line touches a coarser leaf depends only on the surface
+// regions of its two transverse coordinates (BlockGeo::wbmask).
 template <class FT>
-__global__ void __launch_bounds__(kSmrThreads) k_smr_emf_wb(MeshView<FT> m, const int *blocks) {
-    const int b = blocks[blockIdx.y];
-    const BlockGeo &g = m.geo[b];
-    if (m.ts[g.grp].skip) return;
+__device__ __forceinline__ FT smr_edge_read(const MeshView<FT> &m, const BlockGeo &g, const FT *EE,
+                                            int c, size_t idx, const int *j) {
+    const FT *E = EE + (size_t)c * m.nface;
+    const int c1 = mod3(c + 1), c2 = mod3(c + 2);
+    if (((j[c1] | j[c2]) & 1) || j[c] >= m.n[c] + (j[c] & 1)) return E[idx];
+    const int r1 = j[c1] == 0 ? 0 : (j[c1] == m.n[c1] ? 2 : 1);
+    const int r2 = j[c2] == 0 ? 0 : (j[c2] == m.n[c2] ? 2 : 1);
+    if (!((g.wbmask >> ((c * 3 + r1) * 3 + r2)) & 1)) return E[idx];
+    const size_t sh = c == 0 ? 1 : (c == 1 ? (size_t)(m.n[0] + 1) : (size_t)(m.n[0] + 1) * (m.n[1] + 1));
+    const size_t i0 = idx - (j[c] & 1) * sh;
+    const FT dl = (FT)g.dx[c];
+    return (E[i0] * dl + E[i0 + sh] * dl) / (dl + dl);
+}
+
+// act_flx of a stage in one launch: rows below nx restrict the EMFs of the
+// coarser blocks' shared edges (block xblk[row], one thread per edge-box
+// point), the others restrict the fluxes of the faces whose neighbour is
+// finer, tpr tasks per row (ftask[(row - nx) * tpr + k], fmax threads per
+// task).  Both read only what the sweep wrote and write disjoint coarser
+// data.
+template <class FT, int NV>
+__global__ void __launch_bounds__(kSmrThreads) k_smr_restrict(MeshView<FT> m, SmrView s,
+                                                             const int *xblk, int nx,
+                                                             const int2 *ftask, int ntask,
+                                                             int fmax, int tpr) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m.nface) return;
-    const int i[3] = {(int)(t % (m.n[0] + 1)), (int)((t / (m.n[0] + 1)) % (m.n[1] + 1)),
-                      (int)(t / ((size_t)(m.n[0] + 1) * (m.n[1] + 1)))};
-    if (!on_block_surface(m, i)) return;
-    const size_t sh[3] = {1, (size_t)(m.n[0] + 1), (size_t)(m.n[0] + 1) * (m.n[1] + 1)};
-    for (int a = 0; a < 3; ++a) {
-        if (i[a] >= m.n[a] || (i[a] & 1)) continue;
-        const int a1 = mod3(a + 1), a2 = mod3(a + 2);
-        if ((i[a1] & 1) || (i[a2] & 1)) continue;
-        int o1, o2;
-        if (!(edge_levels(m, g, a, i, o1, o2) & 2)) continue;
-        FT *E = m.EE + ((size_t)b * 3 + a) * m.nface + t;
-        const FT dl = (FT)g.dx[a];
-        const FT avg = (E[0] * dl + E[sh[a]] * dl) / (dl + dl);
-        E[0] = avg;
-        E[sh[a]] = avg;
+    if ((int)blockIdx.y < nx) {
+        if constexpr (NV == 8) smr_emf_coarse_at(m, s, xblk[blockIdx.y], t);
+    } else {
+        const int k = (int)(t / fmax);
+        const int task = ((int)blockIdx.y - nx) * tpr + k;
+        if (k >= tpr || task >= ntask) return;
+        smr_flux_at<FT, NV>(m, s, ftask[task], (int)(t - (size_t)k * fmax));
     }
+}
+
+// The update of the blocks of a refined mesh that are not updated by the
+// fused kernel (k_update's arithmetic), with the cross-level corrections
+// of intg_flx folded in: the edges of a finer block on a coarser edge line
+// read as their pair's mean (smr_edge_read), and a cell next to a face
+// whose neighbour is finer adds dt A / V times the restricted flux after
+// the regular update, z sides first, then y, then x
+// (hydro/integrator/integrator.h:313-382), before the state is stored.
+template <class FT, bool MHD>
+__global__ void __launch_bounds__(kThreads) k_smr_update(MeshView<FT> m, int step) {
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
+    const int b = row_block(m);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (MHD ? m.nface : m.ncell)) return;
+    const BlockGeo &g = m.geo[b];
+    update_point<FT, MHD>(
+        m, b, t, step,
+        [&](const FT *EE, int c, size_t idx, const int *i, int sa) -> FT {
+            int j[3] = {i[0], i[1], i[2]};
+            if (sa >= 0) ++j[sa];
+            return smr_edge_read(m, g, EE, c, idx, j);
+        },
+        [&](const int *i, double *uu) {
+            if (!g.rface) return;
+            const double dt = m.ts[g.grp].dt_use * (step == 0 ? 1.0 : 0.5);
+            for (int a = 2; a >= 0; --a)
+                for (int sd = 0; sd <= 1; ++sd) {
+                    if (!((g.rface >> (2 * a + sd)) & 1)) continue;
+                    if (i[a] != (sd ? m.n[a] - 1 : 0)) continue;
+                    int q[3] = {i[0], i[1], i[2]};
+                    q[a] += sd;
+                    const FT *F = m.F[a] + (size_t)b * m.nv * m.nfa[a] + xidx(m.n, a, q);
+                    const FT ds = (FT)g.sfc[a];
+                    const FT dv = (FT)g.vol * (FT)(sd ? -1 : 1);
+                    const double conv = dt * (double)ds / (double)dv;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) uu[v] += conv * (double)F[(size_t)v * m.nfa[a]];
+                }
+        });
 }
 
 }  // namespace kr

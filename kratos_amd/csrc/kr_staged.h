@@ -510,14 +510,17 @@ __global__ void __launch_bounds__(kThreads) k_emf(MeshView<FT> m) {
 // Flux divergence + Heun slot (hydro/integrator/integrator.h:56-68, 156-182,
 // 241-265; rk2.h:20-41) and, for MHD, the CT update of the face field
 // (mhd/integrator/integrator.h:385-462 with rk2.h:191-205).
-template <class FT, bool MHD>
-__global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
-    if (m.ts[m.geo[row_block(m)].grp].skip) return;
+//
+// One point of the update: the cell at face-box point t of block b (when
+// it is a cell) and, for MHD, the faces of that point.  edge(EE, c, idx,
+// i, s) reads the edge EMF of component c at edge-box index idx, EE being
+// block b's edges; the edge sits at the point's coordinates i, one further
+// along axis s when s >= 0.  finish(i, uu) may add to the cell's new state
+// before it is stored.
+template <class FT, bool MHD, class EdgeRead, class CellFinish>
+__device__ __forceinline__ void update_point(const MeshView<FT> &m, int b, size_t t, int step,
+                                             EdgeRead edge, CellFinish finish) {
     constexpr int NV = MHD ? 8 : 5;
-    const int b = row_block(m);
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t nitem = MHD ? m.nface : m.ncell;
-    if (t >= nitem) return;
     int i[3];
     const int e0 = m.n[0] + (MHD ? 1 : 0), e1 = m.n[1] + (MHD ? 1 : 0);
     i[0] = (int)(t % e0);
@@ -570,6 +573,7 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
                 }
             }
         }
+        finish(i, uu);
 #pragma unroll
         for (int v = 0; v < 5; ++v) Ut[(size_t)v * m.ncell] = uu[v];
     }
@@ -591,12 +595,12 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
             const bool th_1 = ax_1 >= m.ndim, th_2 = ax_2 >= m.ndim;
             // FP64 edge circulation (also in the mixed form: the EMF is
             // widened before the product so CT keeps div B at round-off).
-            double db = th_2 ? 0.0 : (double)EE[(size_t)ax_1 * m.nface + t] * g.dx[ax_1];
+            double db = th_2 ? 0.0 : (double)edge(EE, ax_1, t, i, -1) * g.dx[ax_1];
             if (!th_1) {
-                db -= (double)EE[(size_t)ax_2 * m.nface + t] * g.dx[ax_2];
-                db += (double)EE[(size_t)ax_2 * m.nface + t + sh[ax_1]] * g.dx[ax_2];
+                db -= (double)edge(EE, ax_2, t, i, -1) * g.dx[ax_2];
+                db += (double)edge(EE, ax_2, t + sh[ax_1], i, ax_1) * g.dx[ax_2];
             }
-            if (!th_2) db -= (double)EE[(size_t)ax_1 * m.nface + t + sh[ax_2]] * g.dx[ax_1];
+            if (!th_2) db -= (double)edge(EE, ax_1, t + sh[ax_2], i, ax_2) * g.dx[ax_1];
             // the thin face family's area (sfc_thin, integrator.h:74-81)
             const double sfc = a >= m.ndim ? g.dx[ax_1] * g.dx[ax_2] : g.sfc[a];
             db *= (sfc > 0 ? (double)(-dt / sfc) : (double)(0));
@@ -605,6 +609,20 @@ __global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
                 db + (step == 0 ? bs : (bs + Bt[(size_t)a * m.nface]) / 2);
         }
     }
+}
+
+template <class FT, bool MHD>
+__global__ void __launch_bounds__(kThreads) k_update(MeshView<FT> m, int step) {
+    if (m.ts[m.geo[row_block(m)].grp].skip) return;
+    const int b = row_block(m);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (MHD ? m.nface : m.ncell)) return;
+    update_point<FT, MHD>(
+        m, b, t, step,
+        [&](const FT *EE, int c, size_t idx, const int *, int) -> FT {
+            return EE[(size_t)c * m.nface + idx];
+        },
+        [](const int *, double *) {});
 }
 
 // Max |div B| per workgroup (diagnostic): sum over axes of the face
