@@ -13,11 +13,13 @@
 //     position; restricted halos are filled first and the prolongation's
 //     slopes read them (act_bnd's q_ghst stage before q_plgt);
 //   * a face whose neighbour is finer takes the area-weighted mean of the
-//     four finer face fluxes; the regular update skipped it and
-//     k_smr_intg adds it to the adjacent cell (z sides, then y, then x);
+//     four finer face fluxes (k_smr_restrict); the regular update skips it
+//     and k_smr_update adds it to the adjacent cell (z sides, then y, then
+//     x) before the cell is stored;
 //   * an edge of a coarser block that a finer leaf shares takes the
-//     length-weighted mean of the two finer edges along it; then the finer
-//     blocks' edge pairs on such lines are overwritten with that mean.
+//     length-weighted mean of the two finer edges along it
+//     (k_smr_restrict); the finer blocks' edge pairs on such lines take
+//     that mean too, which k_smr_update applies where it reads them.
 #pragma once
 
 #include "kr_staged.h"
