@@ -171,6 +171,10 @@ struct kr_ctx {
     bool fused = false;
     int variant = -1;               // kSweepVariants index of the fused kernel
     int prod = -1;                  // refined mesh: producer tile (0: 16x8, 1: 8x8), -1 none
+    bool mx_list = false;           // k_sweep_mx walks a device work list (ensembles)
+    int4 *d_items = nullptr;        // its items (block, tile, z0, z1) and their count
+    int *d_nitems = nullptr;
+    int slots[16] = {0};            // resident workgroups on the GPU per variant (0: not asked yet)
     std::vector<int> candidates;    // variants the autotuner times
     int tune_next = 0;              // next candidate to time
     bool tuned = false;
@@ -247,7 +251,7 @@ void release(kr_ctx *c) {
     void *bufs[] = {c->U,  c->BF, c->W,       c->F[0],  c->F[1],   c->F[2],  c->EE, c->EC,
                     c->geo, c->ts, c->grp_first, c->dt_part, c->red_part, c->red_out,
                     c->d_table, c->d_fill[0], c->d_fill[1], c->d_ftask, c->d_xblk,
-                    c->d_pure, c->d_pfill, c->d_sweep};
+                    c->d_pure, c->d_pfill, c->d_sweep, c->d_items, c->d_nitems};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->d_table = nullptr;
@@ -255,6 +259,8 @@ void release(kr_ctx *c) {
     c->d_pfill = nullptr;
     c->d_ftask = nullptr;
     c->d_xblk = c->d_pure = c->d_sweep = nullptr;
+    c->d_items = nullptr;
+    c->d_nitems = nullptr;
     c->U = c->BF = nullptr;
     c->W = c->EE = c->EC = nullptr;
     c->F[0] = c->F[1] = c->F[2] = nullptr;
@@ -609,6 +615,16 @@ void commit(kr_ctx *c) {
         if (c->fused) c->variant = c->candidates[0];
         c->tune_next = 0;
         c->tuned = c->candidates.size() <= 1;
+        // an ensemble in the mixed form: k_sweep_mx walks a device work list
+        // of the boxes still running (room for the most tiles, 8x8, in up to
+        // n_z / 8 z-ranges each)
+        if (c->fused && mxmesh && c->ngrp > 1) {
+            c->mx_list = true;
+            const size_t ntile_max = (size_t)(c->n[0] / 8) * (size_t)(c->n[1] / 8);
+            const size_t kz_max = (size_t)std::max(1, c->n[2] / 8);
+            c->d_items = dalloc<int4>(nb * ntile_max * kz_max);
+            c->d_nitems = dalloc<int>(1);
+        }
     }
     // The fused kernel reads W through wnbr.  With physical sides, every
     // direction of a block that crosses one gets a halo block in W: after
@@ -780,10 +796,31 @@ struct Launcher {
             break;
         }
     }
+    // Ensembles: the stage's work list (the tiles of the boxes still
+    // running, split into more z-ranges as fewer remain), then a persistent
+    // grid of as many workgroups as the GPU holds at once walking it.
     template <int TX, int TY, int NT>
     void sweep_mx_t(const MeshView<float> &m, int step, unsigned nb) {
         const unsigned nt = (unsigned)((c->n[0] / TX) * (c->n[1] / TY));
-        hipLaunchKernelGGL((k_sweep_mx<TX, TY, NT>), dim3(nt, nb), dim3(NT), 0, c->stream, m, step);
+        if (!c->mx_list) {
+            hipLaunchKernelGGL((k_sweep_mx<TX, TY, NT, false>), dim3(nt, nb), dim3(NT), 0, c->stream,
+                               m, step, (const int4 *)nullptr, (const int *)nullptr);
+            return;
+        }
+        int &slots = c->slots[c->variant & 15];
+        if (slots == 0) {
+            int dev = 0, ncu = 0, per_cu = 0;
+            KR_HIP(hipGetDevice(&dev));
+            KR_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            KR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(k_sweep_mx<TX, TY, NT, true>), NT, 0));
+            slots = std::max(1, ncu * per_cu);
+        }
+        hipLaunchKernelGGL(k_sweep_list, dim3(1), dim3(1024), 0, c->stream, (const TimeState *)c->ts,
+                           (const BlockGeo *)c->geo, (int)nb, (int)nt, c->n[2], slots, c->d_items,
+                           c->d_nitems);
+        hipLaunchKernelGGL((k_sweep_mx<TX, TY, NT, true>), dim3(slots), dim3(NT), 0, c->stream, m,
+                           step, (const int4 *)c->d_items, (const int *)c->d_nitems);
     }
 
     template <class FT, bool MHD> void stage(int step) {

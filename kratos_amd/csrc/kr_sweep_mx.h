@@ -38,8 +38,19 @@ struct MxTile {
     __device__ static int rg(int x, int y) { return (y + 1) * RX + (x + 1); }
 };
 
-template <int TX, int TY, int NT>
-__global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
+// Grid launch (LIST = false): one workgroup per (tile, block row), marching
+// the whole block, planes [0, n_z).  Work-list launch (LIST = true,
+// ensembles): a persistent grid walks the items (block, tile, z0, z1) that
+// k_sweep_list wrote for the boxes still running, item i on workgroup
+// i mod gridDim.x.  A partial range [z0, z1) repeats the march's lead-in
+// planes z0 - 2 .. z0 and computes the faces and edges at its ends as the
+// neighbouring range does, through the same code and from the same inputs,
+// and updates only its own cells and faces, so a split changes no result.
+// The list form's item loop costs registers; its 256- and 192-thread tiles
+// are held to 3 waves per SIMD (168 VGPRs), the occupancy their LDS allows.
+template <int TX, int TY, int NT, bool LIST>
+__global__ void __launch_bounds__(NT, (LIST && NT <= 256) ? 3 : 1)
+    k_sweep_mx(MeshView<float> m, int step, const int4 *items, const int *n_items) {
     using FT = float;
     using S = TileShape<TX, TY>;
     using M = MxTile<TX, TY>;
@@ -48,15 +59,34 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
     static_assert(M::WZ <= NW, "z pencils on the first waves, one pass");
     static_assert(TX * TY + TX + TY <= NT, "one owned cell / edge face per thread");
     static_assert(M::WR <= NW, "one cell-pass item per thread at most");
-    if (m.ts[m.geo[row_block(m)].grp].skip) return;
+    for (int item = (int)blockIdx.x;; item += (int)gridDim.x) {
+    unsigned row, tile;
+    int z0, z1;
+    if constexpr (LIST) {
+        if (item >= *n_items) break;
+        const int4 it = items[item];
+        row = (unsigned)it.x;
+        tile = (unsigned)it.y;
+        z0 = it.z;
+        z1 = it.w;
+    } else {
+        row = blockIdx.y;
+        tile = blockIdx.x;
+        z0 = 0;
+        z1 = m.n[2];
+    }
+    if (m.ts[m.geo[m.blist ? m.blist[row] : (int)row].grp].skip) {
+        if constexpr (LIST) continue;
+        else return;
+    }
     const int tid = threadIdx.x, lane = tid & 63;
     // the wave's role in a scalar register: the kind branches below are
     // uniform; rtid is the thread's index within the roles
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rtid = wave * 64 + lane;
-    const int b = row_block(m);
+    const int b = m.blist ? m.blist[row] : (int)row;
     const int ntx = m.n[0] / TX;
-    const int x0 = (blockIdx.x % ntx) * TX, y0 = (blockIdx.x / ntx) * TY;
+    const int x0 = (tile % ntx) * TX, y0 = (tile / ntx) * TY;
     const int nz = m.n[2];
     const bool xlast = x0 + TX == m.n[0], ylast = y0 + TY == m.n[1];
     const BlockGeo &g = m.geo[b];
@@ -213,15 +243,15 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
     const bool yedge = ylast && et >= TY && et < TY + TX;
 
     double bvn[3] = {0, 0, 0};       // normal face fields of the next cell pass
-    issue_w(-2, ring3(-2));
-    for (int L = -2; L <= nz + 1; ++L) {
+    issue_w(z0 - 2, ring3(z0 - 2));
+    for (int L = z0 - 2; L <= z1 + 1; ++L) {
         // ------------------------------------------------ 1. plane L arrives
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         const int P = L - 2;
 
         // ------------------------------------------------ 2. fluxes
-        if (L >= 0) {
+        if (L >= z0) {
             const int p = L - 1;
             const int s0 = ring3(p), sm = ring3(p - 1), sp = ring3(p + 1);
             const int fe = ring2(p);
@@ -245,7 +275,7 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
                         qR[v] = q0 - t;
                         qLn[v] = q0 + t;
                     }
-                    if (p >= 0 && p <= nz) {
+                    if (p >= z0 && p <= nz) {
                         FT wl[8], wr[8], flx[8];
 #pragma unroll
                         for (int v = 0; v < 8; ++v) {
@@ -266,7 +296,7 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
                         sFz[1][gz] = flx[BCC + 2];
                         sFz[2][gz] = -flx[BCC + 1];
                         if (zint) {
-                            if (P >= 0) {
+                            if (P >= z0) {
                                 const FT fa = (FT)sGeo[3 + 2];
                                 const size_t cc = cidx(m.n, x0 + cx_, y0 + cy_, P);
                                 double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + cc;
@@ -348,7 +378,7 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         __syncthreads();
 
         // ------------------------------------------------ 3. EMFs
-        if (L >= 1) {
+        if (L >= z0 + 1) {
             const int p = L - 1;
             const int s0 = ring3(p), sm = ring3(p - 1);
             const int fe = ring2(p), fm = ring2(p - 1);
@@ -400,12 +430,12 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
         // iteration were the last to read
         const CpItem cpi = cp_item();
         const bool cp_own = cpi.r < M::NR;
-        const bool cp_now = cp_own && L >= -1 && L <= nz;
+        const bool cp_now = cp_own && L >= z0 - 1 && L <= z1;
         const double bv[3] = {bvn[0], bvn[1], bvn[2]};
-        if (cp_own && L + 1 >= -1 && L + 1 <= nz) cp_load(cpi, L + 1, bvn);
-        if (L + 1 <= nz + 1) issue_w(L + 1, ring3(L + 1));
+        if (cp_own && L + 1 >= z0 - 1 && L + 1 <= z1) cp_load(cpi, L + 1, bvn);
+        if (L + 1 <= z1 + 1) issue_w(L + 1, ring3(L + 1));
 
-        if (L >= 1) {
+        if (L >= z0 + 1) {
             const int q = L - 1;
             const int fq = ring2(q), fpp = ring2(P);
             // Face-field update of one face (mhd/integrator/rk2.h:48-62): its
@@ -443,7 +473,7 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
                 const FaceUpd uz = face_load(2, x, y, q);
                 const FaceUpd ux = face_load(0, x, y, P < 0 ? 0 : P);
                 const FaceUpd uy = face_load(1, x, y, P < 0 ? 0 : P);
-                if (q < nz) {
+                if (q < z1) {
                     const size_t c = cidx(m.n, x0 + x, y0 + y, q);
                     const double *Us = m.U + ((size_t)b * 2 + step) * 5 * nc + c;
                     const double *Ut = m.U + ((size_t)b * 2 + (1 - step)) * 5 * nc + c;
@@ -473,22 +503,25 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) part[v] = uu[v];
                 }
-                face_store(uz, ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
-                                  sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
-                if (P >= 0) {
+                // z face q: the range's own faces, and the block's top face
+                // in the last range
+                if (q < z1 || z1 == nz)
+                    face_store(uz, ct(sEx[fq][S::exi(x, y)], dx, sEy[fq][S::eyi(x, y)], dy,
+                                      sEy[fq][S::eyi(x + 1, y)], sEx[fq][S::exi(x, y + 1)], 2));
+                if (P >= z0) {
                     face_store(ux, ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
                                       sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
                     face_store(uy, ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
                                       sEx[fq][S::exi(x, y)], sEz[fpp][S::ezi(x + 1, y)], 1));
                 }
             }
-            if (P >= 0 && xedge) {
+            if (P >= z0 && xedge) {
                 const int x = TX, y = et;
                 const FaceUpd ux = face_load(0, x, y, P);
                 face_store(ux, ct(sEy[fpp][S::eyi(x, y)], dy, sEz[fpp][S::ezi(x, y)], dz,
                                   sEz[fpp][S::ezi(x, y + 1)], sEy[fq][S::eyi(x, y)], 0));
             }
-            if (P >= 0 && yedge) {
+            if (P >= z0 && yedge) {
                 const int x = et - TY, y = TY;
                 const FaceUpd uy = face_load(1, x, y, P);
                 face_store(uy, ct(sEz[fpp][S::ezi(x, y)], dz, sEx[fpp][S::exi(x, y)], dx,
@@ -496,6 +529,50 @@ __global__ void __launch_bounds__(NT) k_sweep_mx(MeshView<float> m, int step) {
             }
         }
         if (cp_now) cp_finish(cpi, L, bv);
+    }
+    if constexpr (!LIST) break;
+    __syncthreads();          // the next item reuses the LDS
+    }
+}
+
+// Work list of the ensemble's k_sweep_mx launch for this stage: every tile
+// of the blocks whose time group still runs, each split into kz z-ranges
+// of at least 8 planes, kz chosen so that the items cover the resident
+// workgroup slots (more ranges as fewer boxes remain).  One workgroup;
+// items in block, tile, range order.
+__global__ void __launch_bounds__(1024) k_sweep_list(const TimeState *ts, const BlockGeo *geo, int nb,
+                                                     int ntile, int nz, int slots, int4 *items,
+                                                     int *n_items) {
+    __shared__ int part[1024];
+    __shared__ int s_kz;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int per = (nb + nt - 1) / nt, b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
+    int cnt = 0;
+    for (int b = b0; b < b1; ++b) cnt += ts[geo[b].grp].skip ? 0 : 1;
+    part[t] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        int act = 0;
+        for (int i = 0; i < nt; ++i) {      // exclusive prefix of the active counts
+            const int c = part[i];
+            part[i] = act;
+            act += c;
+        }
+        const int kz_max = nz / 8 > 1 ? nz / 8 : 1;
+        int kz = act ? (slots + act * ntile / 2) / (act * ntile) : 1;
+        kz = kz < 1 ? 1 : (kz > kz_max ? kz_max : kz);
+        s_kz = kz;
+        *n_items = act * ntile * kz;
+    }
+    __syncthreads();
+    const int kz = s_kz;
+    int r = part[t];
+    for (int b = b0; b < b1; ++b) {
+        if (ts[geo[b].grp].skip) continue;
+        for (int tl = 0; tl < ntile; ++tl)
+            for (int k = 0; k < kz; ++k)
+                items[((size_t)r * ntile + tl) * kz + k] = make_int4(b, tl, k * nz / kz, (k + 1) * nz / kz);
+        ++r;
     }
 }
 

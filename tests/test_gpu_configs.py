@@ -133,6 +133,42 @@ def test_ensemble_mixed_per_box(gpu_lib):
     assert len(set(cycles)) > 1, "boxes of different beta need different cycle counts"
 
 
+def test_ensemble_mixed_equals_boxes_alone(gpu_lib):
+    """The mixed ensemble's stage kernel walks a device work list of the
+    boxes still running, splitting their columns into z-ranges as fewer
+    remain; a box run alone takes the grid launch.  Boxes are independent,
+    so every box of the ensemble must equal its own run bit for bit."""
+    N = (32, 32, 32)
+    n_box = 6
+    spec = MeshSpec(N, N)
+    phys = Physics(mhd=True, gamma=G, cfl=0.3, t_lim=0.02, top_up=True)
+    ics = [problems.turbulence(N, seed=s, beta_range=(0.1, 10.0)) for s in range(n_box)]
+    print(f"\n[case] ensemble mixed {n_box} x {N} vs boxes alone", flush=True)
+    ctx = kratos_amd.Context(spec, phys, "mixed", members=n_box)
+    for m, (u, b, _) in enumerate(ics):
+        ctx.upload(u, b, member=m)
+    ctx.advance(100000)
+    got = [ctx.download(member=m) + (ctx.time(member=m),) for m in range(n_box)]
+    st = ctx.stats()
+    ctx.close()
+    assert st["sweep_kernel"] == 1, st
+    cycles = []
+    for m, (u, b, _) in enumerate(ics):
+        one = kratos_amd.Context(spec, phys, "mixed", tile=tuple(st["sweep_tile"][0:2]) + (1,))
+        one.upload(u, b)
+        done = one.advance(100000)
+        ou, ob = one.download()
+        ot = one.time()
+        one.close()
+        gu, gb, gt = got[m]
+        assert gt[0] == ot[0] == 0.02 and gt[2] == done, (m, gt, ot, done)
+        cycles.append(gt[2])
+        np.testing.assert_array_equal(gu, ou, err_msg=f"box {m}")
+        for x, y in zip(gb, ob):
+            np.testing.assert_array_equal(x, y, err_msg=f"box {m} faces")
+    assert len(set(cycles)) > 1, "boxes of different beta need different cycle counts"
+
+
 _BLAST = {}
 
 
