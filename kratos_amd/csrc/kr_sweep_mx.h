@@ -535,16 +535,19 @@ __global__ void __launch_bounds__(NT, (LIST && NT <= 256) ? 3 : 1)
     }
 }
 
-// Work list of the ensemble's k_sweep_mx launch for this stage: every tile
-// of the blocks whose time group still runs, each split into kz z-ranges
-// of at least 8 planes, kz chosen so that the items cover the resident
-// workgroup slots (more ranges as fewer boxes remain).  One workgroup;
-// items in block, tile, range order.
+// Work list of a stage's k_sweep_mx launch: every tile of the blocks whose
+// time group still runs, in block, tile, range order.  With T such tiles
+// and S resident workgroup slots, the first T - T mod S tiles march their
+// whole column and the last T mod S are split into kz z-ranges (at least 8
+// planes each, kz about S / (T mod S)), so that the persistent grid's
+// workgroups, taking items i, i + S, ..., all finish about together (when
+// T < S, every tile is split: more ranges as fewer boxes remain).  One
+// workgroup.
 __global__ void __launch_bounds__(1024) k_sweep_list(const TimeState *ts, const BlockGeo *geo, int nb,
                                                      int ntile, int nz, int slots, int4 *items,
                                                      int *n_items) {
     __shared__ int part[1024];
-    __shared__ int s_kz;
+    __shared__ int s_kz, s_full;
     const int t = threadIdx.x, nt = blockDim.x;
     const int per = (nb + nt - 1) / nt, b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
     int cnt = 0;
@@ -558,20 +561,28 @@ __global__ void __launch_bounds__(1024) k_sweep_list(const TimeState *ts, const 
             part[i] = act;
             act += c;
         }
+        const int tiles = act * ntile, rem = tiles % slots;
         const int kz_max = nz / 8 > 1 ? nz / 8 : 1;
-        int kz = act ? (slots + act * ntile / 2) / (act * ntile) : 1;
+        int kz = rem ? (slots + rem / 2) / rem : 1;
         kz = kz < 1 ? 1 : (kz > kz_max ? kz_max : kz);
         s_kz = kz;
-        *n_items = act * ntile * kz;
+        s_full = tiles - rem;
+        *n_items = s_full + rem * kz;
     }
     __syncthreads();
-    const int kz = s_kz;
+    const int kz = s_kz, full = s_full;
     int r = part[t];
     for (int b = b0; b < b1; ++b) {
         if (ts[geo[b].grp].skip) continue;
-        for (int tl = 0; tl < ntile; ++tl)
-            for (int k = 0; k < kz; ++k)
-                items[((size_t)r * ntile + tl) * kz + k] = make_int4(b, tl, k * nz / kz, (k + 1) * nz / kz);
+        for (int tl = 0; tl < ntile; ++tl) {
+            const int k0 = r * ntile + tl;         // the tile's rank among the running tiles
+            if (k0 < full) {
+                items[k0] = make_int4(b, tl, 0, nz);
+            } else {
+                for (int k = 0; k < kz; ++k)
+                    items[full + (k0 - full) * kz + k] = make_int4(b, tl, k * nz / kz, (k + 1) * nz / kz);
+            }
+        }
         ++r;
     }
 }
