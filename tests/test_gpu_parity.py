@@ -217,6 +217,32 @@ def test_fused_tile_variants(gpu_lib, tile, precision):
         assert rel_l1(cat(gb), cat(ob)) <= 1e-5
 
 
+def test_mixed_tile_variants_identical(gpu_lib):
+    """The mixed form's fused kernel on a periodic one-level mesh
+    (k_sweep_mx): every tile the autotuner may pick gives the same state bit
+    for bit, since each face and edge goes through one code path whatever
+    the tiling -- so tuning cycles leave the state a fixed choice would."""
+    N, blk = (64, 64, 64), (32, 32, 32)
+    spec = kratos_amd.MeshSpec(N, blk)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    u, b, _ = problems.turbulence(N, seed=3)
+    ref = None
+    for tile in [(16, 8, 1), (32, 8, 1), (16, 16, 1), (8, 8, 1)]:
+        print(f"\n[case] mixed tile {tile}", flush=True)
+        ctx = kratos_amd.Context(spec, phys, "mixed", path="fused", tile=tile)
+        ctx.upload(u, b)
+        assert ctx.advance(6) == 6
+        assert ctx.stats()["sweep_kernel"] == 1
+        gu, gb = ctx.download()
+        ctx.close()
+        if ref is None:
+            ref = (gu, gb)
+            continue
+        np.testing.assert_array_equal(gu, ref[0], err_msg=str(tile))
+        for x, y in zip(gb, ref[1]):
+            np.testing.assert_array_equal(x, y, err_msg=str(tile))
+
+
 def test_autotuner_keeps_state(gpu_lib):
     """The autotuner runs the first cycles with a different fused-kernel
     variant each; the state must still equal the oracle's bit for bit (FP64),
