@@ -448,6 +448,8 @@ def main():
     ap.add_argument("--no-hydro", action="store_true")
     ap.add_argument("--hydro-only", action="store_true",
                     help="run only the hydro leg and print its JSON object")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the ensemble leg (one JSON line), e.g. under a profiler")
     ap.add_argument("--smr-only", action="store_true",
                     help="run only the refined-mesh blast leg and print its JSON")
     ap.add_argument("--smr-root", type=int, default=128)
@@ -469,6 +471,11 @@ def main():
     if args.hydro_only:
         hyd = hydro_leg(args.n, args.block, args.steps, args.warmup, args.path)
         print(json.dumps({k: (float(v) if isinstance(v, np.floating) else v) for k, v in hyd.items()}))
+        return
+
+    if args.ensemble_only:
+        ens = ensemble_leg(args.ens_boxes, args.ens_n, args.t_ens, args.path)
+        print(json.dumps({k: (float(v) if isinstance(v, np.floating) else v) for k, v in ens.items()}))
         return
 
     if args.smr_only:
