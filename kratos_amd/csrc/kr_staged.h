@@ -297,10 +297,13 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
                 b_max[a] = pmax(kfabs(b_l), kfabs(b_r));
             }
             e_m /= 2;
+            const double e_in = cons[ENE];
             cons[ENE] -= e_m;
             c2p(cons, prim, e, hr, he);
             cons[ENE] += e_m;
-            U[(size_t)ENE * m.ncell] = cons[ENE];
+            // the subtract / add round trip (and a healed pressure) can move
+            // the stored energy; where it did not, the slot already holds it
+            if (cons[ENE] != e_in) U[(size_t)ENE * m.ncell] = cons[ENE];
             if (hr) U[0] = cons[RHO];
 #pragma unroll
             for (int v = 0; v < 5; ++v) W[(size_t)v * m.ncell] = (FT)prim[v];
