@@ -816,7 +816,8 @@ struct Launcher {
                 &per_cu, reinterpret_cast<const void *>(k_sweep_mx<TX, TY, NT, true>), NT, 0));
             slots = std::max(1, ncu * per_cu);
         }
-        hipLaunchKernelGGL(k_sweep_list, dim3(1), dim3(1024), 0, c->stream, (const TimeState *)c->ts,
+        const unsigned lt = (unsigned)std::min<size_t>(1024, ceil_div((size_t)nb, 64) * 64);
+        hipLaunchKernelGGL(k_sweep_list, dim3(1), dim3(lt), 0, c->stream, (const TimeState *)c->ts,
                            (const BlockGeo *)c->geo, (int)nb, (int)nt, c->n[2], slots, c->d_items,
                            c->d_nitems);
         hipLaunchKernelGGL((k_sweep_mx<TX, TY, NT, true>), dim3(slots), dim3(NT), 0, c->stream, m,

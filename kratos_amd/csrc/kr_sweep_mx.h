@@ -556,7 +556,8 @@ __global__ void __launch_bounds__(1024) k_sweep_list(const TimeState *ts, const 
     __syncthreads();
     if (t == 0) {
         int act = 0;
-        for (int i = 0; i < nt; ++i) {      // exclusive prefix of the active counts
+        const int used = (nb + per - 1) / per;   // threads that own blocks
+        for (int i = 0; i < used; ++i) {         // exclusive prefix of the active counts
             const int c = part[i];
             part[i] = act;
             act += c;
