@@ -84,6 +84,34 @@ def test_smr_16_blocks_mhd_fp64(gpu_lib):
     assert divb <= 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_smr_32_blocks_mhd(gpu_lib, precision):
+    """32^3 blocks (the block size of the refined blast's outer levels):
+    one root block of eight refined, so every level-0 block meets the
+    level-1 octant across a face, an edge or a corner (periodic wrap
+    included).  FP64 bit-identical; mixed: density L1 <= 1e-5 relative."""
+    spec = RefinedMeshSpec((64, 64, 64), (32, 32, 32),
+                           refine=[(1, (0.3, 0.3, 0.3), (0.45, 0.45, 0.45))])
+    assert spec.n_blocks == 15
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    g, o, gt, ot, divb = _run(spec, phys, problems.smooth_blocks(spec, seed=12, mhd=True), 3,
+                              precision)
+    assert divb <= 1e-12
+    if precision == "fp64":
+        assert gt[0] == ot[0]
+        _assert_equal(spec, g, o, True)
+        return
+    num = den = 0.0
+    for b in range(spec.n_blocks):
+        _, dx = spec.geometry(b)
+        v = dx[0] * dx[1] * dx[2]
+        gr = np.asarray(g[b][0]).reshape(-1, 8)[:, 0]
+        orr = np.asarray(o[b][0]).reshape(-1, 8)[:, 0]
+        num += np.abs(gr - orr).sum() * v
+        den += np.abs(orr).sum() * v
+    assert num / den <= 1e-5
+
+
 def test_smr_three_levels_mhd_fp64(gpu_lib):
     spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=THREE)
     assert spec.n_levels == 3
