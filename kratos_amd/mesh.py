@@ -568,6 +568,10 @@ class Context:
                     self.h, "kr_get_time")
         return t.value, dt.value, seed.value
 
+    def set_time(self, t, dt_seed):
+        """Time of every group and the dt seed of the next cycle (kr_set_time)."""
+        _capi.check(self.lib.kr_set_time(self.h, float(t), float(dt_seed)), self.h, "kr_set_time")
+
     def divb_max(self):
         v = C.c_double()
         _capi.check(self.lib.kr_divb_max(self.h, C.byref(v)), self.h, "kr_divb_max")

@@ -243,6 +243,77 @@ def test_mixed_tile_variants_identical(gpu_lib):
             np.testing.assert_array_equal(x, y, err_msg=str(tile))
 
 
+def test_mixed_autotuner_keeps_state(gpu_lib):
+    """Mixed form, one-level mesh: the autotuner's first cycles run a
+    different tile shape each, eagerly, before the graph is captured; the
+    state and time equal a run with the tile fixed, bit for bit."""
+    N, blk = (64, 64, 64), (32, 32, 32)
+    spec = kratos_amd.MeshSpec(N, blk)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    u, b, _ = problems.turbulence(N, seed=6)
+    print("\n[case] mixed autotuned, 8 cycles", flush=True)
+    ctx = kratos_amd.Context(spec, phys, "mixed", path="fused")
+    ctx.upload(u, b)
+    assert ctx.advance(2) == 2
+    assert ctx.step() > 0
+    assert ctx.advance(5) == 5
+    st = ctx.stats()
+    assert st["sweep_kernel"] == 1
+    assert len([t for t in st["tune_ms"] if t > 0]) == 4, st["tune_ms"]
+    gu, gb = ctx.download()
+    t = ctx.time()
+    ctx.close()
+    print("[case] mixed 16x8 fixed, 8 cycles", flush=True)
+    ref = kratos_amd.Context(spec, phys, "mixed", path="fused", tile=(16, 8, 1))
+    ref.upload(u, b)
+    assert ref.advance(8) == 8
+    ru, rb = ref.download()
+    assert ref.time() == t
+    ref.close()
+    np.testing.assert_array_equal(gu, ru)
+    for x, y in zip(gb, rb):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_mixed_resume_after_upload(gpu_lib):
+    """Mixed form, one-level mesh: a downloaded state, uploaded into a new
+    context with the same time and dt seed, continues bit for bit as the
+    uninterrupted run: every stage starts from the stored state (c2p, then
+    the stage kernel), so the state between cycles is all a restart needs."""
+    N, blk = (64, 64, 64), (32, 32, 32)
+    spec = kratos_amd.MeshSpec(N, blk)
+    phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
+    u, b, _ = problems.turbulence(N, seed=7)
+
+    def ctx():
+        return kratos_amd.Context(spec, phys, "mixed", path="fused", tile=(16, 8, 1))
+
+    print("\n[case] mixed 7 cycles uninterrupted", flush=True)
+    c = ctx()
+    c.upload(u, b)
+    assert c.advance(7) == 7
+    ref = c.download()
+    t_ref = c.time()
+    c.close()
+    print("[case] mixed 3 cycles, download, upload, 4 cycles", flush=True)
+    c = ctx()
+    c.upload(u, b)
+    assert c.advance(3) == 3
+    mid = c.download()
+    t, _, seed = c.time()
+    c.close()
+    c = ctx()
+    c.upload(*mid)
+    c.set_time(t, seed)
+    assert c.advance(4) == 4
+    out = c.download()
+    assert c.time()[0] == t_ref[0]
+    c.close()
+    np.testing.assert_array_equal(out[0], ref[0])
+    for x, y in zip(out[1], ref[1]):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_autotuner_keeps_state(gpu_lib):
     """The autotuner runs the first cycles with a different fused-kernel
     variant each; the state must still equal the oracle's bit for bit (FP64),
