@@ -277,7 +277,10 @@ def smr_leg(n_root, nb, steps, warmup, cycles_acc):
     on |x_a| <= 0.125 (tree-balanced), all blocks one size; both forms on the
     staged kernels with the cross-level halo, flux and EMF corrections.
     Reports each form's cell-updates/s over `steps` replayed cycles, and the
-    mixed form's density L1 against FP64 after `cycles_acc` cycles."""
+    mixed form's density L1 against FP64 after `cycles_acc` cycles; then the
+    same mesh given with blocks of two sizes (groups of eight leaves as
+    blocks of 2 nb), mixed form: its rate and whether its state after
+    `cycles_acc` cycles equals the one-size mesh's bit for bit."""
     import kratos_amd
     from kratos_amd import problems
     spec = kratos_amd.RefinedMeshSpec((n_root,) * 3, (nb,) * 3,
@@ -285,7 +288,11 @@ def smr_leg(n_root, nb, steps, warmup, cycles_acc):
                                               (2, (-0.125,) * 3, (0.125,) * 3)],
                                       x_min=(-0.5,) * 3, x_max=(0.5,) * 3)
     phys = kratos_amd.Physics(mhd=True, gamma=5.0 / 3.0, cfl=0.3)
-    blocks = problems.blast_blocks(spec)
+    # the same mesh with blocks of two sizes: every group of eight same-level
+    # leaves at an even index as one block of 2 nb (its state is kept with
+    # one value per shared face, as such a block holds it)
+    msp = kratos_amd.MergedMeshSpec(spec)
+    blocks = msp.split_blocks(msp.merge_blocks(problems.blast_blocks(spec)))
     out = {"workload": f"MHD blast, {n_root}^3 root cells, {spec.n_blocks} blocks of {nb}^3 on "
                        f"{spec.n_levels} levels", "blocks": spec.n_blocks,
            "blocks_per_level": [sum(1 for lv in spec.leaves if lv[0] == L)
@@ -315,6 +322,26 @@ def smr_leg(n_root, nb, steps, warmup, cycles_acc):
         out[f"kernel_ms_per_cycle_{form}"] = {
             k: round(v * prof["kernel_calls"][k] / 2, 4) for k, v in prof["kernel_ms"].items()}
         ctx.close()
+    # the two-size mesh, mixed form: same cycles, same state bit for bit
+    ctx = kratos_amd.Context(msp, phys, "mixed")
+    ctx.upload_blocks(msp.merge_blocks(blocks))
+    ctx.advance(cycles_acc)
+    sync_all(ctx)
+    same = all(np.array_equal(np.asarray(x[0]), np.asarray(y[0]))
+               for x, y in zip(msp.split_blocks(ctx.download_blocks()), states["mixed"]))
+    ctx.advance(warmup)
+    settle(ctx)
+    sync_all(ctx)
+    t0 = time.perf_counter()
+    done = ctx.advance(steps)
+    sync_all(ctx)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    n2 = sum(msp.merged(b) for b in range(msp.n_blocks))
+    out["two_block_sizes"] = {
+        "blocks": msp.n_blocks, f"blocks_of_{2 * nb}^3": n2, f"blocks_of_{nb}^3": msp.n_blocks - n2,
+        "cell_updates_per_s_mixed": spec.n_cells * done / wall,
+        "state_after_cycles_acc_equals_one_size_mesh": bool(same)}
     num = den = 0.0
     for b in range(spec.n_blocks):
         _, dx = spec.geometry(b)

@@ -88,7 +88,17 @@ typedef struct kr_block_desc {
                               independent boxes uses 0 .. n_box-1, and blocks
                               of different groups never neighbour.         */
     int    reg[3];         /* logical block index at the block's level       */
-    int    n_ceff[3];      /* cells per axis (== kr_config.n_cell_block)     */
+    int    n_ceff[3];      /* cells per axis: kr_config.n_cell_block, or
+                              twice it on every axis -- a block that covers
+                              the eight blocks reg + {0,1}^3 of its level
+                              (reg is the lowest; xf0 its corner, dx0 its
+                              cell).  The library runs it as those eight, so
+                              a mesh with such blocks takes its topology from
+                              level and reg (as a refined mesh does) and
+                              gives the same cells as the mesh of eight.
+                              The reference has one block size per mesh
+                              (mesh/meshgen/meshgen.cpp:446-448); this is an
+                              extension of the descriptor, not a counterpart. */
     double xf0[3], dx0[3]; /* geometry, as meshgen computes it               */
     int    nbr[27];        /* neighbour block id per direction
                               ((dz+1)*3 + (dy+1))*3 + (dx+1); -1 = physical  */
@@ -140,7 +150,8 @@ int kr_add_blocks(kr_ctx *ctx, int n, const kr_block_desc *blocks);
 /* Copy one block's state (slot 0) host -> device.  Replaces
  * dual_t::copy_h2d (mesh/block/dual_data.cpp:46-49 ->
  * hydro/types/hyd_data.cpp:59-63, mhd/types/fld_data.cpp:97-101).
- * bf_aos is ignored for hydro.                                           */
+ * bf_aos is ignored for hydro.  Arrays are of the block's own size
+ * (n_ceff), also for a block of twice n_cell_block.                      */
 int kr_upload(kr_ctx *ctx, int id, const double *u_aos, const double *bf_aos);
 
 /* Copy one block's state (slot 0) device -> host.  Replaces the D2H of

@@ -7,7 +7,7 @@ initial conditions.  There is no CPU fallback: without the built library
 every entry point raises ``KratosError``.
 """
 from ._capi import KratosError, LIB_PATH, load  # noqa: F401
-from .mesh import Context, MeshSpec, Physics, RefinedMeshSpec  # noqa: F401
+from .mesh import Context, MergedMeshSpec, MeshSpec, Physics, RefinedMeshSpec  # noqa: F401
 from . import problems  # noqa: F401
 
 __all__ = ["Context", "MeshSpec", "Physics", "KratosError", "load", "problems", "LIB_PATH"]

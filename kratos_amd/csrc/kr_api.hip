@@ -10,12 +10,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
 #include <mutex>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -166,6 +168,14 @@ struct kr_ctx {
     std::string err;
     std::vector<kr_block_desc> descs;
     std::unordered_map<int, int> id2idx;
+    // A caller's block of twice n_cell_block on every axis is kept as its
+    // eight octants, blocks of n_cell_block one level-L index apart: part[b]
+    // is -1 for a block the caller gave, 8 m + o for octant o of merged
+    // block m (o's bits: x, y, z half)
+    std::vector<int> part;
+    std::vector<int> merged_id;             // m -> caller's id
+    std::unordered_map<int, int> mid2m;     // caller's id -> m
+    std::vector<std::array<int, 8>> oct;    // m -> internal index of each octant
     bool committed = false;
     bool mhd = false, mixed = false;
     bool fused = false;
@@ -497,10 +507,26 @@ void commit(kr_ctx *c) {
     const size_t nb = c->descs.size();
     // internal order: by time group (stable), so each group's blocks and
     // CFL partials are contiguous
-    std::stable_sort(c->descs.begin(), c->descs.end(),
-                     [](const kr_block_desc &a, const kr_block_desc &b) { return a.group < b.group; });
+    {
+        std::vector<int> perm(nb);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int a, int b) { return c->descs[a].group < c->descs[b].group; });
+        std::vector<kr_block_desc> d(nb);
+        std::vector<int> p(nb);
+        for (size_t b = 0; b < nb; ++b) {
+            d[b] = c->descs[perm[b]];
+            p[b] = c->part[perm[b]];
+        }
+        c->descs.swap(d);
+        c->part.swap(p);
+    }
     c->id2idx.clear();
-    for (size_t b = 0; b < nb; ++b) c->id2idx[c->descs[b].id] = (int)b;
+    c->oct.assign(c->merged_id.size(), {});
+    for (size_t b = 0; b < nb; ++b) {
+        if (c->part[b] < 0) c->id2idx[c->descs[b].id] = (int)b;
+        else c->oct[c->part[b] / 8][c->part[b] % 8] = (int)b;
+    }
     c->ngrp = c->descs.back().group + 1;
     std::vector<int> first(c->ngrp + 1, 0);
     for (size_t b = 0; b < nb; ++b) first[c->descs[b].group + 1] += 1;
@@ -519,7 +545,9 @@ void commit(kr_ctx *c) {
         c->nfa[ax] = s;
     }
 
-    bool multi_level = false;
+    // refined meshes, and meshes with merged blocks, take their topology
+    // from level and logical index
+    bool multi_level = !c->merged_id.empty();
     for (auto &d : c->descs) multi_level |= d.level != 0;
     std::vector<BlockGeo> g(nb);
     for (size_t b = 0; b < nb; ++b) {
@@ -1095,6 +1123,115 @@ double *noalloc_scratch() {
 
 }  // namespace
 
+namespace {
+
+// One block of n_cell_block cells from the reference's host layout into
+// slot 0 of the pools: u_aos interior cells, bf_aos the ghosted face array.
+void upload_one(kr_ctx *c, size_t b, const double *u_aos, const double *bf_aos) {
+    const int nvar = c->mhd ? 8 : 5;
+    std::vector<double> &s = c->stage;
+    s.assign(5 * c->ncell, 0.0);
+    for (size_t cc = 0; cc < c->ncell; ++cc)
+        for (int v = 0; v < 5; ++v) s[v * c->ncell + cc] = u_aos[cc * nvar + v];
+    KR_HIP(hipMemcpy(c->U + b * 2 * 5 * c->ncell, s.data(), 5 * c->ncell * sizeof(double),
+                     hipMemcpyHostToDevice));
+    if (c->mhd) {
+        // host face array: n + (n > 1) faces per axis plus two ghost layers
+        // each side (mhd/types/fld_data.cpp:18-26); a thin axis has one
+        const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
+                           c->n[2] + (c->n[2] > 1)};
+        const int N0 = nf[0] + 4, N1 = nf[1] + 4;
+        s.assign(3 * c->nface, 0.0);
+        for (int k = 0; k < nf[2]; ++k)
+            for (int j = 0; j < nf[1]; ++j)
+                for (int i = 0; i < nf[0]; ++i) {
+                    const size_t src = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
+                    const size_t f = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                    for (int a = 0; a < 3; ++a) s[a * c->nface + f] = bf_aos[src + a];
+                }
+        KR_HIP(hipMemcpy(c->BF + b * 2 * 3 * c->nface, s.data(), 3 * c->nface * sizeof(double),
+                         hipMemcpyHostToDevice));
+    }
+}
+
+// One block of n_cell_block cells back into the reference's host layout
+// (cell-centred B from the face average; faces with zero ghost layers).
+void download_one(kr_ctx *c, size_t b, double *u_aos, double *bf_aos) {
+    const int nvar = c->mhd ? 8 : 5;
+    std::vector<double> s(5 * c->ncell);
+    KR_HIP(hipMemcpy(s.data(), c->U + b * 2 * 5 * c->ncell, 5 * c->ncell * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (size_t cc = 0; cc < c->ncell; ++cc)
+        for (int v = 0; v < 5; ++v) u_aos[cc * nvar + v] = s[v * c->ncell + cc];
+    if (c->mhd) {
+        std::vector<double> f(3 * c->nface);
+        KR_HIP(hipMemcpy(f.data(), c->BF + b * 2 * 3 * c->nface, 3 * c->nface * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        // a thin axis has one face, both sides of the cell
+        const size_t sh[3] = {c->n[0] > 1 ? (size_t)1 : 0,
+                              c->n[1] > 1 ? (size_t)(c->n[0] + 1) : 0,
+                              c->n[2] > 1 ? (size_t)(c->n[0] + 1) * (c->n[1] + 1) : 0};
+        for (int k = 0; k < c->n[2]; ++k)
+            for (int j = 0; j < c->n[1]; ++j)
+                for (int i = 0; i < c->n[0]; ++i) {
+                    const size_t cc = ((size_t)k * c->n[1] + j) * c->n[0] + i;
+                    const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                    for (int a = 0; a < 3; ++a)
+                        u_aos[cc * nvar + BCC + a] =
+                            (f[a * c->nface + f0] + f[a * c->nface + f0 + sh[a]]) / 2;
+                }
+        if (bf_aos) {
+            const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
+                               c->n[2] + (c->n[2] > 1)};
+            const int N0 = nf[0] + 4, N1 = nf[1] + 4, N2 = nf[2] + 4;
+            std::memset(bf_aos, 0, (size_t)N0 * N1 * N2 * 3 * sizeof(double));
+            for (int k = 0; k < nf[2]; ++k)
+                for (int j = 0; j < nf[1]; ++j)
+                    for (int i = 0; i < nf[0]; ++i) {
+                        const size_t dst = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
+                        const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
+                        for (int a = 0; a < 3; ++a) bf_aos[dst + a] = f[a * c->nface + f0];
+                    }
+        }
+    }
+}
+
+// Host arrays of a block of 2n cells per axis <-> its octant o's arrays of
+// n cells: cells [h n, h n + n) and faces [h n, h n + n] per axis, h the
+// octant's bit for that axis.  `to_oct` copies from the block to the octant.
+void octant_copy(const kr_ctx *c, int o, bool to_oct, double *big_u, double *big_bf, double *u,
+                 double *bf) {
+    const int nvar = c->mhd ? 8 : 5;
+    const int *n = c->n;
+    int h[3];
+    for (int a = 0; a < 3; ++a) h[a] = ((o >> a) & 1) * n[a];
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const size_t so = (((size_t)k * n[1] + j) * n[0] + i) * nvar;
+                const size_t sb = (((size_t)(k + h[2]) * 2 * n[1] + (j + h[1])) * 2 * n[0] + (i + h[0])) * nvar;
+                for (int v = 0; v < nvar; ++v) {
+                    if (to_oct) u[so + v] = big_u[sb + v];
+                    else big_u[sb + v] = u[so + v];
+                }
+            }
+    if (!c->mhd || !bf || !big_bf) return;
+    // ghosted face arrays: n + 1 faces per axis plus two ghost layers a side
+    const int M0 = n[0] + 5, M1 = n[1] + 5, B0 = 2 * n[0] + 5, B1 = 2 * n[1] + 5;
+    for (int k = 0; k <= n[2]; ++k)
+        for (int j = 0; j <= n[1]; ++j)
+            for (int i = 0; i <= n[0]; ++i) {
+                const size_t so = (((size_t)(k + 2) * M1 + (j + 2)) * M0 + (i + 2)) * 3;
+                const size_t sb = (((size_t)(k + h[2] + 2) * B1 + (j + h[1] + 2)) * B0 + (i + h[0] + 2)) * 3;
+                for (int a = 0; a < 3; ++a) {
+                    if (to_oct) bf[so + a] = big_bf[sb + a];
+                    else big_bf[sb + a] = bf[so + a];
+                }
+            }
+}
+
+}  // namespace
+
 extern "C" {
 
 int kr_create(const kr_config *cfg, kr_ctx **out) {
@@ -1161,15 +1298,41 @@ int kr_add_blocks(kr_ctx *c, int n, const kr_block_desc *blocks) {
         if (n < 0 || (n > 0 && !blocks)) throw std::invalid_argument("bad block list");
         for (int i = 0; i < n; ++i) {
             const kr_block_desc &d = blocks[i];
-            for (int a = 0; a < 3; ++a)
-                if (d.n_ceff[a] != c->cfg.n_cell_block[a])
-                    throw std::invalid_argument("block size differs from n_cell_block");
+            bool same = true, twice = true;
+            for (int a = 0; a < 3; ++a) {
+                same = same && d.n_ceff[a] == c->cfg.n_cell_block[a];
+                twice = twice && d.n_ceff[a] == 2 * c->cfg.n_cell_block[a];
+            }
+            if (!same && !twice)
+                throw std::invalid_argument("block size is neither n_cell_block nor twice it on every axis");
             if (d.level < 0 || d.level > 7)
                 throw std::invalid_argument("refinement level out of 0..7");
             if (d.group < 0 || d.group >= (1 << 20)) throw std::invalid_argument("bad time group");
-            if (c->id2idx.count(d.id)) throw std::invalid_argument("duplicate block id");
-            c->id2idx[d.id] = (int)c->descs.size();
-            c->descs.push_back(d);
+            if (c->id2idx.count(d.id) || c->mid2m.count(d.id))
+                throw std::invalid_argument("duplicate block id");
+            if (same) {
+                c->id2idx[d.id] = (int)c->descs.size();
+                c->descs.push_back(d);
+                c->part.push_back(-1);
+                continue;
+            }
+            // twice n_cell_block: its eight octants at the block's level,
+            // reg the index of the lowest one; neighbours from level and reg
+            const int m = (int)c->merged_id.size();
+            c->merged_id.push_back(d.id);
+            c->mid2m[d.id] = m;
+            for (int o = 0; o < 8; ++o) {
+                kr_block_desc od = d;
+                for (int a = 0; a < 3; ++a) {
+                    const int h = (o >> a) & 1;
+                    od.n_ceff[a] = c->cfg.n_cell_block[a];
+                    od.reg[a] = d.reg[a] + h;
+                    od.xf0[a] = d.xf0[a] + h * (c->cfg.n_cell_block[a] * d.dx0[a]);
+                }
+                for (int k = 0; k < 27; ++k) od.nbr[k] = -1;
+                c->descs.push_back(od);
+                c->part.push_back(8 * m + o);
+            }
         }
         return KR_OK;
     });
@@ -1178,33 +1341,22 @@ int kr_add_blocks(kr_ctx *c, int n, const kr_block_desc *blocks) {
 int kr_upload(kr_ctx *c, int id, const double *u_aos, const double *bf_aos) {
     return guarded(c, [&] {
         commit(c);
-        auto it = c->id2idx.find(id);
-        if (it == c->id2idx.end()) throw std::invalid_argument("unknown block id");
         if (!u_aos || (c->mhd && !bf_aos)) throw std::invalid_argument("null state pointer");
-        const size_t b = it->second;
-        const int nvar = c->mhd ? 8 : 5;
-        std::vector<double> &s = c->stage;
-        s.assign(5 * c->ncell, 0.0);
-        for (size_t cc = 0; cc < c->ncell; ++cc)
-            for (int v = 0; v < 5; ++v) s[v * c->ncell + cc] = u_aos[cc * nvar + v];
-        KR_HIP(hipMemcpy(c->U + b * 2 * 5 * c->ncell, s.data(), 5 * c->ncell * sizeof(double),
-                         hipMemcpyHostToDevice));
-        if (c->mhd) {
-            // host face array: n + (n > 1) faces per axis plus two ghost layers
-            // each side (mhd/types/fld_data.cpp:18-26); a thin axis has one
-            const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
-                               c->n[2] + (c->n[2] > 1)};
-            const int N0 = nf[0] + 4, N1 = nf[1] + 4;
-            s.assign(3 * c->nface, 0.0);
-            for (int k = 0; k < nf[2]; ++k)
-                for (int j = 0; j < nf[1]; ++j)
-                    for (int i = 0; i < nf[0]; ++i) {
-                        const size_t src = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
-                        const size_t f = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
-                        for (int a = 0; a < 3; ++a) s[a * c->nface + f] = bf_aos[src + a];
-                    }
-            KR_HIP(hipMemcpy(c->BF + b * 2 * 3 * c->nface, s.data(), 3 * c->nface * sizeof(double),
-                             hipMemcpyHostToDevice));
+        auto mi = c->mid2m.find(id);
+        if (mi != c->mid2m.end()) {
+            // a block of twice n_cell_block: each octant from its part
+            const int nvar = c->mhd ? 8 : 5;
+            std::vector<double> u(c->ncell * nvar), bf;
+            if (c->mhd) bf.assign((size_t)(c->n[0] + 5) * (c->n[1] + 5) * (c->n[2] + 5) * 3, 0.0);
+            for (int o = 0; o < 8; ++o) {
+                octant_copy(c, o, true, const_cast<double *>(u_aos), const_cast<double *>(bf_aos),
+                            u.data(), c->mhd ? bf.data() : nullptr);
+                upload_one(c, (size_t)c->oct[mi->second][o], u.data(), c->mhd ? bf.data() : nullptr);
+            }
+        } else {
+            auto it = c->id2idx.find(id);
+            if (it == c->id2idx.end()) throw std::invalid_argument("unknown block id");
+            upload_one(c, (size_t)it->second, u_aos, bf_aos);
         }
         return KR_OK;
     });
@@ -1213,47 +1365,30 @@ int kr_upload(kr_ctx *c, int id, const double *u_aos, const double *bf_aos) {
 int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
     return guarded(c, [&] {
         commit(c);
-        auto it = c->id2idx.find(id);
-        if (it == c->id2idx.end()) throw std::invalid_argument("unknown block id");
         if (!u_aos) throw std::invalid_argument("null state pointer");
+        auto mi = c->mid2m.find(id);
+        auto it = c->id2idx.find(id);
+        if (mi == c->mid2m.end() && it == c->id2idx.end())
+            throw std::invalid_argument("unknown block id");
         KR_HIP(hipStreamSynchronize(c->stream));
-        const size_t b = it->second;
-        const int nvar = c->mhd ? 8 : 5;
-        std::vector<double> s(5 * c->ncell);
-        KR_HIP(hipMemcpy(s.data(), c->U + b * 2 * 5 * c->ncell, 5 * c->ncell * sizeof(double),
-                         hipMemcpyDeviceToHost));
-        for (size_t cc = 0; cc < c->ncell; ++cc)
-            for (int v = 0; v < 5; ++v) u_aos[cc * nvar + v] = s[v * c->ncell + cc];
-        if (c->mhd) {
-            std::vector<double> f(3 * c->nface);
-            KR_HIP(hipMemcpy(f.data(), c->BF + b * 2 * 3 * c->nface, 3 * c->nface * sizeof(double),
-                             hipMemcpyDeviceToHost));
-            // a thin axis has one face, both sides of the cell
-            const size_t sh[3] = {c->n[0] > 1 ? (size_t)1 : 0,
-                                  c->n[1] > 1 ? (size_t)(c->n[0] + 1) : 0,
-                                  c->n[2] > 1 ? (size_t)(c->n[0] + 1) * (c->n[1] + 1) : 0};
-            for (int k = 0; k < c->n[2]; ++k)
-                for (int j = 0; j < c->n[1]; ++j)
-                    for (int i = 0; i < c->n[0]; ++i) {
-                        const size_t cc = ((size_t)k * c->n[1] + j) * c->n[0] + i;
-                        const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
-                        for (int a = 0; a < 3; ++a)
-                            u_aos[cc * nvar + BCC + a] =
-                                (f[a * c->nface + f0] + f[a * c->nface + f0 + sh[a]]) / 2;
-                    }
-            if (bf_aos) {
-                const int nf[3] = {c->n[0] + (c->n[0] > 1), c->n[1] + (c->n[1] > 1),
-                                   c->n[2] + (c->n[2] > 1)};
-                const int N0 = nf[0] + 4, N1 = nf[1] + 4, N2 = nf[2] + 4;
-                std::memset(bf_aos, 0, (size_t)N0 * N1 * N2 * 3 * sizeof(double));
-                for (int k = 0; k < nf[2]; ++k)
-                    for (int j = 0; j < nf[1]; ++j)
-                        for (int i = 0; i < nf[0]; ++i) {
-                            const size_t dst = ((size_t)((k + 2) * N1 + (j + 2)) * N0 + (i + 2)) * 3;
-                            const size_t f0 = ((size_t)k * (c->n[1] + 1) + j) * (c->n[0] + 1) + i;
-                            for (int a = 0; a < 3; ++a) bf_aos[dst + a] = f[a * c->nface + f0];
-                        }
+        if (mi != c->mid2m.end()) {
+            // a block of twice n_cell_block, gathered from its octants; a face
+            // two octants share is taken from the upper one (both hold it
+            // bit for bit)
+            const int nvar = c->mhd ? 8 : 5;
+            std::vector<double> u(c->ncell * nvar), bf;
+            if (c->mhd && bf_aos) {
+                bf.assign((size_t)(c->n[0] + 5) * (c->n[1] + 5) * (c->n[2] + 5) * 3, 0.0);
+                std::memset(bf_aos, 0,
+                            (size_t)(2 * c->n[0] + 5) * (2 * c->n[1] + 5) * (2 * c->n[2] + 5) * 3 * sizeof(double));
             }
+            for (int o = 0; o < 8; ++o) {
+                download_one(c, (size_t)c->oct[mi->second][o], u.data(), bf.empty() ? nullptr : bf.data());
+                octant_copy(c, o, false, u_aos, bf.empty() ? nullptr : bf_aos, u.data(),
+                            bf.empty() ? nullptr : bf.data());
+            }
+        } else {
+            download_one(c, (size_t)it->second, u_aos, bf_aos);
         }
         return KR_OK;
     });

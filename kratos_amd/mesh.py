@@ -425,6 +425,128 @@ class RefinedMeshSpec:
         raise TypeError("refined meshes take per-block arrays (upload_blocks)")
 
 
+class MergedMeshSpec:
+    """A refined mesh given with blocks of two sizes: every group of eight
+    leaves ``reg + {0,1}^3`` of one level, ``reg`` even on each axis, is one
+    block of twice the block size (the eight's level, ``reg`` the lowest
+    one's index, the lowest one's corner and cell), the other leaves stay
+    blocks of the base size.  Blocks follow the base mesh's leaf order, a
+    merged block at its lowest leaf's place; ``parts[b]`` lists the base
+    leaves of block b (octant o = x + 2 y + 4 z bits).  The reference has
+    one block size per mesh (mesh/meshgen/meshgen.cpp:446-448); the library
+    runs a merged block as its eight leaves, so the cells are the base
+    mesh's."""
+
+    def __init__(self, base):
+        self.base = base
+        self.n_cell_block = base.n_cell_block
+        self.x_min, self.x_max, self.bnd = base.x_min, base.x_max, base.bnd
+        self.periodic = base.periodic
+        self.n_levels = base.n_levels
+        where = {leaf: i for i, leaf in enumerate(base.leaves)}
+        used, self.parts = set(), []
+        for i, leaf in enumerate(base.leaves):
+            if i in used:
+                continue
+            L, r = leaf[0], leaf[1:]
+            kids = None
+            if all(v % 2 == 0 for v in r):
+                kids = [where.get((L,) + tuple(r[a] + ((o >> a) & 1) for a in range(3)))
+                        for o in range(8)]
+                if any(k is None for k in kids):
+                    kids = None
+            if kids:
+                used.update(kids)
+                self.parts.append(kids)
+            else:
+                self.parts.append([i])
+
+    @property
+    def n_blocks(self):
+        return len(self.parts)
+
+    @property
+    def n_cells(self):
+        return self.base.n_cells
+
+    def merged(self, b):
+        return len(self.parts[b]) == 8
+
+    def block_cells(self, b):
+        f = 2 if self.merged(b) else 1
+        return tuple(f * v for v in self.n_cell_block)
+
+    def block_descs(self, members=1):
+        if members != 1:
+            raise ValueError("refined meshes have one member")
+        arr = (_capi.KrBlockDesc * self.n_blocks)()
+        for b in range(self.n_blocks):
+            d, lo = arr[b], self.parts[b][0]
+            d.id, d.level, d.group = b, self.base.level(lo), 0
+            xf0, dx0 = self.base.geometry(lo)
+            nb = self.block_cells(b)
+            for a in range(3):
+                d.reg[a] = self.base.reg(lo)[a]
+                d.n_ceff[a] = nb[a]
+                d.xf0[a], d.dx0[a] = xf0[a], dx0[a]
+            for k in range(27):
+                d.nbr[k] = -1          # derived from level and reg
+        return arr
+
+    def _octant(self, o):
+        n = self.n_cell_block
+        return [((o >> a) & 1) * n[a] for a in range(3)]
+
+    def merge_blocks(self, blocks):
+        """Per-leaf (u_aos, bf_aos) of the base mesh -> per-block arrays;
+        a face two leaves share is taken from the upper leaf."""
+        n = self.n_cell_block
+        out = []
+        for parts in self.parts:
+            if len(parts) == 1:
+                out.append(blocks[parts[0]])
+                continue
+            nv = np.asarray(blocks[parts[0]][0]).size // (n[0] * n[1] * n[2])
+            U = np.zeros((2 * n[2], 2 * n[1], 2 * n[0], nv))
+            mhd = blocks[parts[0]][1] is not None
+            B = np.zeros((2 * n[2] + 5, 2 * n[1] + 5, 2 * n[0] + 5, 3)) if mhd else None
+            for o, leaf in enumerate(parts):
+                h = self._octant(o)
+                U[h[2]:h[2] + n[2], h[1]:h[1] + n[1], h[0]:h[0] + n[0]] = \
+                    np.asarray(blocks[leaf][0]).reshape(n[2], n[1], n[0], nv)
+                if mhd:
+                    bo = np.asarray(blocks[leaf][1]).reshape(n[2] + 5, n[1] + 5, n[0] + 5, 3)
+                    B[2 + h[2]:3 + h[2] + n[2], 2 + h[1]:3 + h[1] + n[1], 2 + h[0]:3 + h[0] + n[0]] = \
+                        bo[2:3 + n[2], 2:3 + n[1], 2:3 + n[0]]
+            out.append((U.ravel(), B.ravel() if mhd else None))
+        return out
+
+    def split_blocks(self, blocks):
+        """Per-block arrays -> per-leaf arrays of the base mesh (zero ghost
+        layers, as kr_download gives them)."""
+        n = self.n_cell_block
+        out = [None] * self.base.n_blocks
+        for b, parts in enumerate(self.parts):
+            if len(parts) == 1:
+                out[parts[0]] = blocks[b]
+                continue
+            ua, ba = blocks[b]
+            nv = np.asarray(ua).size // (8 * n[0] * n[1] * n[2])
+            U = np.asarray(ua).reshape(2 * n[2], 2 * n[1], 2 * n[0], nv)
+            B = None if ba is None else np.asarray(ba).reshape(2 * n[2] + 5, 2 * n[1] + 5, 2 * n[0] + 5, 3)
+            for o, leaf in enumerate(parts):
+                h = self._octant(o)
+                u = U[h[2]:h[2] + n[2], h[1]:h[1] + n[1], h[0]:h[0] + n[0]].copy()
+                bo = None
+                if B is not None:
+                    bo = np.zeros((n[2] + 5, n[1] + 5, n[0] + 5, 3))
+                    bo[2:3 + n[2], 2:3 + n[1], 2:3 + n[0]] = \
+                        B[2 + h[2]:3 + h[2] + n[2], 2 + h[1]:3 + h[1] + n[1], 2 + h[0]:3 + h[0] + n[0]]
+                    bo = bo.ravel()
+                out[leaf] = (u.ravel(), bo)
+        return out
+
+
 @dataclass
 class Physics:
     mhd: bool = True
@@ -532,10 +654,10 @@ class Context:
             _capi.check(self.lib.kr_upload(self.h, b, ua.ctypes.data, bp), self.h, "kr_upload")
 
     def download_blocks(self):
-        n = self.spec.n_cell_block
         nvar = 8 if self.mhd else 5
         out = []
         for b in range(self.spec.n_blocks):
+            n = self.spec.block_cells(b) if hasattr(self.spec, "block_cells") else self.spec.n_cell_block
             ua = np.zeros(n[0] * n[1] * n[2] * nvar)
             ba = np.zeros(bf_host_size(n)) if self.mhd else None
             _capi.check(self.lib.kr_download(self.h, b, ua.ctypes.data,

@@ -4,6 +4,8 @@ every conserved value and face field bit-identical; mixed form: density L1
 <= 1e-5 relative.  Cases cover hydro and MHD, two and three levels, a
 physical (outflow / reflecting) hydro box, and a refined mesh whose blocks
 all sit on level 0 (the same-level path fed by the refined-mesh builder).
+Blocks of two sizes: a mesh whose groups of eight same-level leaves are
+given as blocks of twice the size gives the leaves' states bit for bit.
 MHD runs the fused kernel on blocks with no cross-level direction and its
 producer form (fluxes and EMFs into HBM) plus the staged update on the
 others (and everything staged when asked); 8^3 blocks take the 8x8
@@ -14,7 +16,7 @@ import pytest
 import kratos_amd
 import oracle
 from kratos_amd import problems
-from kratos_amd.mesh import Physics, RefinedMeshSpec, divb_rel_blocks
+from kratos_amd.mesh import MergedMeshSpec, Physics, RefinedMeshSpec, divb_rel_blocks
 
 pytestmark = pytest.mark.gpu
 G = 5.0 / 3.0
@@ -110,6 +112,73 @@ def test_smr_32_blocks_mhd(gpu_lib, precision):
         num += np.abs(gr - orr).sum() * v
         den += np.abs(orr).sum() * v
     assert num / den <= 1e-5
+
+
+def _merged_run(m, phys, blocks, cycles, precision="fp64", path="auto"):
+    print(f"\n[case] merged blocks={m.n_blocks} (leaves {m.base.n_blocks}) mhd={phys.mhd} "
+          f"prec={precision} path={path} cycles={cycles}", flush=True)
+    ctx = kratos_amd.Context(m, phys, precision, path=path)
+    ctx.upload_blocks(m.merge_blocks(blocks))
+    done = ctx.advance(cycles)
+    g = m.split_blocks(ctx.download_blocks())
+    t = ctx.time()
+    ctx.close()
+    return g, t, done
+
+
+@pytest.mark.parametrize("mhd,path", [(True, "auto"), (True, "staged"), (False, "auto")])
+def test_merged_blocks_fp64(gpu_lib, mhd, path):
+    """Blocks of two sizes in one mesh: the two-level mesh with its groups
+    of eight same-level leaves given as blocks of 16^3 beside blocks of 8^3.
+    FP64 bit for bit against the oracle on the leaves (the same cells)."""
+    base = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=TWO)
+    m = MergedMeshSpec(base)
+    assert 0 < sum(m.merged(b) for b in range(m.n_blocks)) < m.n_blocks
+    phys = Physics(mhd=mhd, gamma=G, cfl=0.3)
+    # one value per shared face, as a merged block holds it
+    blocks = m.split_blocks(m.merge_blocks(problems.smooth_blocks(base, seed=13, mhd=mhd)))
+    g, gt, done = _merged_run(m, phys, blocks, 4, path=path)
+    o = oracle.OracleMesh(base, phys, n_threads=16)
+    o.upload_blocks(blocks)
+    assert o.advance(4) == done == 4
+    assert gt[0] == o.time()[0]
+    _assert_equal(base, g, o.download_blocks(), mhd)
+    if mhd:
+        assert divb_rel_blocks(base, g) <= 1e-12
+
+
+def test_merged_blast_mixed_equals_leaves(gpu_lib):
+    """The 3-level blast in the mixed form with its full groups of eight
+    given as blocks of 16^3: the same state, bit for bit, as the mesh of
+    8^3 leaves."""
+    base = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=BLAST3, x_min=(-0.5,) * 3,
+                           x_max=(0.5,) * 3)
+    m = MergedMeshSpec(base)
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    blocks = m.split_blocks(m.merge_blocks(problems.smooth_blocks(base, seed=14, mhd=True)))
+    g, gt, done = _merged_run(m, phys, blocks, 5, "mixed")
+    print(f"[case] leaves blocks={base.n_blocks} mixed cycles=5", flush=True)
+    ctx = kratos_amd.Context(base, phys, "mixed")
+    ctx.upload_blocks(blocks)
+    assert ctx.advance(5) == done
+    ref = ctx.download_blocks()
+    assert ctx.time() == gt
+    ctx.close()
+    _assert_equal(base, g, ref, True)
+
+
+def test_block_size_neither_n_nor_twice_rejected(gpu_lib):
+    base = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=TWO)
+
+    class Bad(MergedMeshSpec):
+        def block_descs(self, members=1):
+            d = super().block_descs(members)
+            b = next(b for b in range(self.n_blocks) if self.merged(b))
+            d[b].n_ceff[2] = 8                   # 16 x 16 x 8: neither size
+            return d
+
+    with pytest.raises(kratos_amd.KratosError, match="neither n_cell_block nor twice"):
+        kratos_amd.Context(Bad(base), Physics(mhd=True, gamma=G, cfl=0.3))
 
 
 def test_smr_three_levels_mhd_fp64(gpu_lib):
