@@ -107,7 +107,8 @@ typedef struct kr_block_desc {
 typedef struct kr_stats {
     double gpu_ms;         /* device time of the last kr_step / kr_advance   */
     int    cycles;         /* cycles performed by that call                  */
-    int    n_blocks;
+    int    n_blocks;       /* blocks on the device (a block of twice
+                              n_cell_block counts as its eight)             */
     long long n_cells;     /* cells per cycle (all blocks)                   */
     double kernel_ms[8];   /* mean launch time per kernel kind (kr_profile)  */
     long long kernel_calls[8];
