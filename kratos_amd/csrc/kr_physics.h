@@ -127,6 +127,26 @@ __device__ __forceinline__ double dt_mhd(const double *v_p, const double *bc_2,
     return dt * cfl;
 }
 
+// The same dt for the mixed form, still in FP64: one reciprocal of rho in
+// place of the seven divisions by it (the values differ from dt_mhd's by
+// rounding; the FP64 form keeps dt_mhd's expressions for bit parity).
+__device__ __forceinline__ double dt_mhd_mixed(const double *v_p, const double *bc_2,
+                                               const double *b_m, const double *dx, int ndim,
+                                               double gamma, double cfl) {
+    const double ri = 1 / v_p[RHO];
+    const double cs_2 = v_p[ENE] * gamma * ri;
+    double dt = (double)FLT_MAX;
+    for (int a = 0; a < ndim; ++a) {
+        const double v_ax2 = b_m[a] * b_m[a] * ri;
+        const double v_t2 = (bc_2[mod3(a + 1)] + bc_2[mod3(a + 2)]) * ri;
+        const double q_2 = v_ax2 + v_t2 + cs_2;
+        const double r_2 = v_ax2 + v_t2 - cs_2;
+        const double c = ksqrt(0.5 * (q_2 + ksqrt(r_2 * r_2 + 4 * cs_2 * v_t2)));
+        dt = pmin(dt, dx[a] / (c + kfabs(v_p[MOM + a])));
+    }
+    return dt * cfl;
+}
+
 // ---------------------------------------------------------------------
 // PLM on a uniform mesh.  hydro/reconstruct/plm.h:116-153.  The FP32 form
 // rescales the slopes by their largest magnitude (:128-137) to keep the

@@ -317,7 +317,9 @@ __global__ void __launch_bounds__(kThreads) k_c2p(MeshView<FT> m, int step) {
                 E[m.ncell] = B[2] * V[0] - B[0] * V[2];
                 E[2 * m.ncell] = B[0] * V[1] - B[1] * V[0];
             }
-            if (step == 0) dtc = dt_mhd(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl);
+            if (step == 0)
+                dtc = std::is_same_v<FT, float> ? dt_mhd_mixed(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl)
+                                                : dt_mhd(prim, bc_2, b_max, g.dx, m.ndim, m.gamma, m.cfl);
         } else {
             const double cs2 = c2p(cons, prim, e, hr, he);
             if (hr) U[0] = cons[RHO];
