@@ -1198,9 +1198,10 @@ void download_one(kr_ctx *c, size_t b, double *u_aos, double *bf_aos) {
 
 // Host arrays of a block of 2n cells per axis <-> its octant o's arrays of
 // n cells: cells [h n, h n + n) and faces [h n, h n + n] per axis, h the
-// octant's bit for that axis.  `to_oct` copies from the block to the octant.
-void octant_copy(const kr_ctx *c, int o, bool to_oct, double *big_u, double *big_bf, double *u,
-                 double *bf) {
+// octant's bit for that axis.  `to_oct`: from the block's arrays (src) into
+// the octant's (dst); otherwise from the octant's (src) into the block's.
+void octant_copy(const kr_ctx *c, int o, bool to_oct, const double *src_u, const double *src_bf,
+                 double *dst_u, double *dst_bf) {
     const int nvar = c->mhd ? 8 : 5;
     const int *n = c->n;
     int h[3];
@@ -1210,12 +1211,10 @@ void octant_copy(const kr_ctx *c, int o, bool to_oct, double *big_u, double *big
             for (int i = 0; i < n[0]; ++i) {
                 const size_t so = (((size_t)k * n[1] + j) * n[0] + i) * nvar;
                 const size_t sb = (((size_t)(k + h[2]) * 2 * n[1] + (j + h[1])) * 2 * n[0] + (i + h[0])) * nvar;
-                for (int v = 0; v < nvar; ++v) {
-                    if (to_oct) u[so + v] = big_u[sb + v];
-                    else big_u[sb + v] = u[so + v];
-                }
+                const size_t from = to_oct ? sb : so, to = to_oct ? so : sb;
+                for (int v = 0; v < nvar; ++v) dst_u[to + v] = src_u[from + v];
             }
-    if (!c->mhd || !bf || !big_bf) return;
+    if (!c->mhd || !src_bf || !dst_bf) return;
     // ghosted face arrays: n + 1 faces per axis plus two ghost layers a side
     const int M0 = n[0] + 5, M1 = n[1] + 5, B0 = 2 * n[0] + 5, B1 = 2 * n[1] + 5;
     for (int k = 0; k <= n[2]; ++k)
@@ -1223,10 +1222,8 @@ void octant_copy(const kr_ctx *c, int o, bool to_oct, double *big_u, double *big
             for (int i = 0; i <= n[0]; ++i) {
                 const size_t so = (((size_t)(k + 2) * M1 + (j + 2)) * M0 + (i + 2)) * 3;
                 const size_t sb = (((size_t)(k + h[2] + 2) * B1 + (j + h[1] + 2)) * B0 + (i + h[0] + 2)) * 3;
-                for (int a = 0; a < 3; ++a) {
-                    if (to_oct) bf[so + a] = big_bf[sb + a];
-                    else big_bf[sb + a] = bf[so + a];
-                }
+                const size_t from = to_oct ? sb : so, to = to_oct ? so : sb;
+                for (int a = 0; a < 3; ++a) dst_bf[to + a] = src_bf[from + a];
             }
 }
 
@@ -1349,8 +1346,7 @@ int kr_upload(kr_ctx *c, int id, const double *u_aos, const double *bf_aos) {
             std::vector<double> u(c->ncell * nvar), bf;
             if (c->mhd) bf.assign((size_t)(c->n[0] + 5) * (c->n[1] + 5) * (c->n[2] + 5) * 3, 0.0);
             for (int o = 0; o < 8; ++o) {
-                octant_copy(c, o, true, const_cast<double *>(u_aos), const_cast<double *>(bf_aos),
-                            u.data(), c->mhd ? bf.data() : nullptr);
+                octant_copy(c, o, true, u_aos, bf_aos, u.data(), c->mhd ? bf.data() : nullptr);
                 upload_one(c, (size_t)c->oct[mi->second][o], u.data(), c->mhd ? bf.data() : nullptr);
             }
         } else {
@@ -1384,8 +1380,8 @@ int kr_download(kr_ctx *c, int id, double *u_aos, double *bf_aos) {
             }
             for (int o = 0; o < 8; ++o) {
                 download_one(c, (size_t)c->oct[mi->second][o], u.data(), bf.empty() ? nullptr : bf.data());
-                octant_copy(c, o, false, u_aos, bf.empty() ? nullptr : bf_aos, u.data(),
-                            bf.empty() ? nullptr : bf.data());
+                octant_copy(c, o, false, u.data(), bf.empty() ? nullptr : bf.data(), u_aos,
+                            bf.empty() ? nullptr : bf_aos);
             }
         } else {
             download_one(c, (size_t)it->second, u_aos, bf_aos);
