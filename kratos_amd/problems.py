@@ -9,7 +9,7 @@ same seed gives the same state on every machine.
 * ``turbulence`` -- seeded Fourier-mode velocity plus a face-centred curl of
   a seeded edge vector potential (div B = 0 to round-off), rescaled to the
   requested rms Mach number and plasma beta (configs 2, 3, 5).
-* ``blast``      -- MHD blast wave in a uniform oblique field (config 4).
+* ``blast``      -- MHD blast wave in a uniform oblique field (the refined-mesh blast).
 * ``orszag_tang``, ``brio_wu`` -- the 2-D and 1-D MHD classics.
 """
 import math
