@@ -296,6 +296,27 @@ def test_smr_mhd_physical_sides_fp64(gpu_lib, bnd):
     assert divb <= 1e-12
 
 
+def test_merged_blocks_physical_sides_fp64(gpu_lib):
+    """Blocks of two sizes on a refined MHD mesh with physical sides (the
+    staged kernels throughout): the groups of eight leaves as blocks of 16^3,
+    some of them on the sides; FP64 bit for bit against the oracle."""
+    bnd = ("out", "fre", "ref", "ref", "fre", "out")
+    base = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=WALL, bnd=bnd)
+    m = MergedMeshSpec(base)
+    on_side = [b for b in range(m.n_blocks) if m.merged(b) and
+               any(base.reg(m.parts[b][0])[a] == 0 for a in range(3))]
+    assert on_side
+    phys = Physics(mhd=True, gamma=G, cfl=0.3)
+    blocks = m.split_blocks(m.merge_blocks(problems.smooth_blocks(base, seed=15)))
+    g, gt, done = _merged_run(m, phys, blocks, 4)
+    o = oracle.OracleMesh(base, phys, n_threads=16)
+    o.upload_blocks(blocks)
+    assert o.advance(4) == done == 4
+    assert gt[0] == o.time()[0]
+    _assert_equal(base, g, o.download_blocks(), True)
+    assert divb_rel_blocks(base, g) <= 1e-12
+
+
 def test_smr_mhd_physical_sides_mixed(gpu_lib):
     spec = RefinedMeshSpec((32, 32, 32), (8, 8, 8), refine=WALL,
                            bnd=("out", "fre", "ref", "ref", "fre", "out"))
